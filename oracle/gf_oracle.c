@@ -1,0 +1,840 @@
+/*
+ * gf_oracle.c -- CPU FP64 restatement of Gaugefields.jl's SU(3) gauge-link hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library; the HIP product never uses it.
+ *
+ * Every function names the reference file:line it restates (paths relative to the
+ * Gaugefields.jl v1.0.5 source tree).  The restatement follows the reference's formulas,
+ * not its rounding order.  It is pinned by tests/test_oracle.py against: the reference's
+ * instanton plaquette golden (test/init.jl:351-371, test/sun_embedded_instanton.jl:145),
+ * cold-start identities, scipy.linalg.expm / expm_frechet, central finite differences of
+ * the potential (force, stout force), a numpy whole-lattice brute force, and gauge
+ * invariance / covariance.
+ *
+ * Layouts are the reference's Julia layouts (see gf_oracle.h).
+ */
+#include "gf_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+typedef struct {
+    cplx e[3][3];
+} m3;
+
+/* ------------------------------------------------------------------------- */
+/* 3x3 complex algebra                                                        */
+/* ------------------------------------------------------------------------- */
+static inline m3 m3_zero(void) {
+    m3 r;
+    memset(&r, 0, sizeof r);
+    return r;
+}
+static inline m3 m3_id(void) {
+    m3 r = m3_zero();
+    r.e[0][0] = r.e[1][1] = r.e[2][2] = 1.0;
+    return r;
+}
+static inline m3 mul(m3 a, m3 b) { /* a b */
+    m3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            r.e[i][j] = a.e[i][0] * b.e[0][j] + a.e[i][1] * b.e[1][j] + a.e[i][2] * b.e[2][j];
+    return r;
+}
+static inline m3 mul_nd(m3 a, m3 b) { /* a b^dag */
+    m3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            r.e[i][j] = a.e[i][0] * conj(b.e[j][0]) + a.e[i][1] * conj(b.e[j][1]) +
+                        a.e[i][2] * conj(b.e[j][2]);
+    return r;
+}
+static inline m3 mul_dn(m3 a, m3 b) { /* a^dag b */
+    m3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            r.e[i][j] = conj(a.e[0][i]) * b.e[0][j] + conj(a.e[1][i]) * b.e[1][j] +
+                        conj(a.e[2][i]) * b.e[2][j];
+    return r;
+}
+static inline m3 dag(m3 a) {
+    m3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r.e[i][j] = conj(a.e[j][i]);
+    return r;
+}
+static inline m3 add(m3 a, m3 b) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) a.e[i][j] += b.e[i][j];
+    return a;
+}
+static inline m3 scale(cplx s, m3 a) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) a.e[i][j] *= s;
+    return a;
+}
+static inline cplx trace(m3 a) { return a.e[0][0] + a.e[1][1] + a.e[2][2]; }
+
+/* column-major load/store of one matrix (Julia (3,3) slice) */
+static inline m3 ld(const double *p) {
+    m3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r.e[i][j] = CMPLX(p[2 * (i + 3 * j)], p[2 * (i + 3 * j) + 1]);
+    return r;
+}
+static inline void st(double *p, m3 a) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            p[2 * (i + 3 * j)] = creal(a.e[i][j]);
+            p[2 * (i + 3 * j) + 1] = cimag(a.e[i][j]);
+        }
+}
+
+/* ------------------------------------------------------------------------- */
+/* lattice geometry                                                           */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int n[4];
+    long V;
+    long *up; /* up[mu*V + s] = s + mu-hat */
+    long *dn; /* dn[mu*V + s] = s - mu-hat */
+} lat;
+
+/* periodic shift, shifted_U! (gaugefields_4D_nowing.jl:380-412) */
+static lat lat_make(const int *dims) {
+    lat L;
+    L.V = 1;
+    for (int m = 0; m < 4; m++) {
+        L.n[m] = dims[m];
+        L.V *= dims[m];
+    }
+    L.up = (long *)malloc(sizeof(long) * 4 * L.V);
+    L.dn = (long *)malloc(sizeof(long) * 4 * L.V);
+    for (long s = 0; s < L.V; s++) {
+        int x[4];
+        long r = s;
+        for (int m = 0; m < 4; m++) {
+            x[m] = (int)(r % L.n[m]);
+            r /= L.n[m];
+        }
+        for (int mu = 0; mu < 4; mu++) {
+            for (int d = -1; d <= 1; d += 2) {
+                int y[4] = {x[0], x[1], x[2], x[3]};
+                y[mu] = (y[mu] + d + L.n[mu]) % L.n[mu];
+                long t = y[0] + (long)L.n[0] * (y[1] + (long)L.n[1] * (y[2] + (long)L.n[2] * y[3]));
+                if (d > 0)
+                    L.up[mu * L.V + s] = t;
+                else
+                    L.dn[mu * L.V + s] = t;
+            }
+        }
+    }
+    return L;
+}
+static void lat_free(lat *L) {
+    free(L->up);
+    free(L->dn);
+}
+#define UP(L, mu, s) ((L)->up[(long)(mu) * (L)->V + (s)])
+#define DN(L, mu, s) ((L)->dn[(long)(mu) * (L)->V + (s)])
+#define LINK(U, L, mu, s) ld((U) + 18 * ((long)(mu) * (L)->V + (s)))
+#define LINKP(U, L, mu, s) ((U) + 18 * ((long)(mu) * (L)->V + (s)))
+
+static int g_threads = 0;
+void gfo_set_threads(int n) { g_threads = n; }
+int gfo_get_threads(void) {
+#ifdef _OPENMP
+    return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+#ifdef _OPENMP
+#define OMP_NT num_threads(gfo_get_threads())
+#endif
+
+/* deterministic reduction: per-thread static blocks, summed in thread order */
+typedef double (*site_fn)(const lat *L, long s, const void *ctx);
+static double lattice_sum(const lat *L, site_fn f, const void *ctx) {
+    int nt = gfo_get_threads();
+    double *part = (double *)calloc((size_t)nt, sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel OMP_NT
+#endif
+    {
+        int tid = 0, n = 1;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+        n = omp_get_num_threads();
+#endif
+        long lo = L->V * tid / n, hi = L->V * (tid + 1) / n;
+        double acc = 0.0;
+        for (long s = lo; s < hi; s++) acc += f(L, s, ctx);
+        part[tid] = acc;
+    }
+    double tot = 0.0;
+    for (int t = 0; t < nt; t++) tot += part[t];
+    free(part);
+    return tot;
+}
+
+/* ------------------------------------------------------------------------- */
+/* TA projection and su(3) basis                                              */
+/* ------------------------------------------------------------------------- */
+static const double SR3I = 0.57735026918962576451; /* 1/sqrt(3) */
+
+/* Traceless_antihermitian_add! for NC=3 (TA_gaugefields_4D_serial.jl:181-269):
+ * c_a += factor * coefficient of TA(v) in the basis i lambda_a / 2. */
+static inline void ta_add(m3 v, double factor, double *c) {
+    double i11 = cimag(v.e[0][0]), i22 = cimag(v.e[1][1]), i33 = cimag(v.e[2][2]);
+    double tri = (i11 + i22 + i33) / 3.0;
+    cplx y11 = I * (i11 - tri), y22 = I * (i22 - tri), y33 = I * (i33 - tri);
+    cplx x12 = v.e[0][1] - conj(v.e[1][0]);
+    cplx x13 = v.e[0][2] - conj(v.e[2][0]);
+    cplx x23 = v.e[1][2] - conj(v.e[2][1]);
+    cplx y12 = 0.5 * x12, y13 = 0.5 * x13, y23 = 0.5 * x23;
+    cplx y21 = -0.5 * conj(x12), y31 = -0.5 * conj(x13), y32 = -0.5 * conj(x23);
+    c[0] += (cimag(y12) + cimag(y21)) * factor;
+    c[1] += (creal(y12) - creal(y21)) * factor;
+    c[2] += (cimag(y11) - cimag(y22)) * factor;
+    c[3] += (cimag(y13) + cimag(y31)) * factor;
+    c[4] += (creal(y13) - creal(y31)) * factor;
+    c[5] += (cimag(y23) + cimag(y32)) * factor;
+    c[6] += (creal(y23) - creal(y32)) * factor;
+    c[7] += SR3I * (cimag(y11) + cimag(y22) - 2.0 * cimag(y33)) * factor;
+}
+
+/* Traceless_antihermitian! as a matrix (gaugefields_4D_nowing.jl:1253-1343):
+ * (M - M^dag)/2 - tr(M - M^dag)/6 */
+static inline m3 ta_mat(m3 v) {
+    double tri = (cimag(v.e[0][0]) + cimag(v.e[1][1]) + cimag(v.e[2][2])) / 3.0;
+    m3 r;
+    for (int i = 0; i < 3; i++) {
+        r.e[i][i] = I * (cimag(v.e[i][i]) - tri);
+        for (int j = i + 1; j < 3; j++) {
+            cplx x = 0.5 * (v.e[i][j] - conj(v.e[j][i]));
+            r.e[i][j] = x;
+            r.e[j][i] = -conj(x);
+        }
+    }
+    return r;
+}
+
+/* Hermitian H = (t/2) sum_a c_a lambda_a, element layout of the legacy exptU!
+ * (TA_gaugefields_4D_serial.jl:790-840: v1..v18 built from c1..c8) */
+static inline m3 herm_from_coeffs(const double *c, double t) {
+    double c1 = 0.5 * t * c[0], c2 = 0.5 * t * c[1], c3 = 0.5 * t * c[2], c4 = 0.5 * t * c[3];
+    double c5 = 0.5 * t * c[4], c6 = 0.5 * t * c[5], c7 = 0.5 * t * c[6], c8 = 0.5 * t * c[7];
+    m3 h;
+    h.e[0][0] = c3 + SR3I * c8;
+    h.e[0][1] = CMPLX(c1, -c2);
+    h.e[0][2] = CMPLX(c4, -c5);
+    h.e[1][0] = CMPLX(c1, c2);
+    h.e[1][1] = -c3 + SR3I * c8;
+    h.e[1][2] = CMPLX(c6, -c7);
+    h.e[2][0] = CMPLX(c4, c5);
+    h.e[2][1] = CMPLX(c6, c7);
+    h.e[2][2] = -2.0 * SR3I * c8;
+    return h;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Cayley-Hamilton exponential (Morningstar & Peardon, PRD 69 054501 (2004))  */
+/* ------------------------------------------------------------------------- */
+/* calc_coefficients_Q (AbstractGaugefields.jl:3284-3343) for Hermitian traceless Q:
+ * exp(iQ) = f0 + f1 Q + f2 Q^2 and, when b != NULL, the derivative coefficients
+ * b[0][j] = d f_j / d c1, b[1][j] = d f_j / d c0 (the reference's b1j, b2j).
+ * The reference evaluates the formulas directly for any sign of c0; here the published
+ * algorithm's c0 -> -c0 symmetry is used (f_j(-c0) = (-1)^j f_j*(c0),
+ * b_ij(-c0) = (-1)^(i+j+1) b_ij*(c0)) so that 9u^2 - w^2 >= 2 c1 never vanishes, and
+ * sin(w)/w, cos(w)/w^2 - sin(w)/w^3 use their Taylor series for |w| < 0.05. */
+static void ch_coeffs(m3 Q, cplx f[3], cplx b[2][3]) {
+    m3 Q2 = mul(Q, Q);
+    double c1 = 0.5 * creal(trace(Q2));
+    cplx det = Q.e[0][0] * (Q.e[1][1] * Q.e[2][2] - Q.e[1][2] * Q.e[2][1]) -
+               Q.e[0][1] * (Q.e[1][0] * Q.e[2][2] - Q.e[1][2] * Q.e[2][0]) +
+               Q.e[0][2] * (Q.e[1][0] * Q.e[2][1] - Q.e[1][1] * Q.e[2][0]);
+    double c0 = creal(det);
+    int neg = c0 < 0.0;
+    double c0a = fabs(c0);
+    double c0max = 2.0 * pow(c1 / 3.0, 1.5);
+    double ratio = c0max > 0.0 ? c0a / c0max : 1.0;
+    if (ratio > 1.0) ratio = 1.0;
+    double th = acos(ratio);
+    double u = sqrt(c1 / 3.0) * cos(th / 3.0);
+    double w = sqrt(c1) * sin(th / 3.0);
+    double w2 = w * w, u2 = u * u;
+    double xi0, xi1;
+    if (fabs(w) < 0.05) {
+        xi0 = 1.0 - w2 / 6.0 * (1.0 - w2 / 20.0 * (1.0 - w2 / 42.0));
+        xi1 = -1.0 / 3.0 + w2 / 30.0 * (1.0 - w2 / 28.0 * (1.0 - w2 / 54.0));
+    } else {
+        xi0 = sin(w) / w;
+        xi1 = cos(w) / w2 - sin(w) / (w2 * w);
+    }
+    double cw = cos(w);
+    cplx emiu = cexp(-I * u), e2iu = cexp(2.0 * I * u);
+    cplx h0 = (u2 - w2) * e2iu + emiu * (8.0 * u2 * cw + 2.0 * I * u * (3.0 * u2 + w2) * xi0);
+    cplx h1 = 2.0 * u * e2iu - emiu * (2.0 * u * cw - I * (3.0 * u2 - w2) * xi0);
+    cplx h2 = e2iu - emiu * (cw + 3.0 * I * u * xi0);
+    double den = 9.0 * u2 - w2;
+    f[0] = h0 / den;
+    f[1] = h1 / den;
+    f[2] = h2 / den;
+    if (b) {
+        cplx r1[3], r2[3];
+        r1[0] = 2.0 * (u + I * (u2 - w2)) * e2iu +
+                2.0 * emiu *
+                    (4.0 * u * (2.0 - I * u) * cw + I * (9.0 * u2 + w2 - I * u * (3.0 * u2 + w2)) * xi0);
+        r1[1] = 2.0 * (1.0 + 2.0 * I * u) * e2iu +
+                emiu * (-2.0 * (1.0 - I * u) * cw + I * (6.0 * u + I * (w2 - 3.0 * u2)) * xi0);
+        r1[2] = 2.0 * I * e2iu + I * emiu * (cw - 3.0 * (1.0 - I * u) * xi0);
+        r2[0] = -2.0 * e2iu + 2.0 * I * u * emiu * (cw + (1.0 + 4.0 * I * u) * xi0 + 3.0 * u2 * xi1);
+        r2[1] = -I * emiu * (cw + (1.0 + 2.0 * I * u) * xi0 - 3.0 * u2 * xi1);
+        r2[2] = emiu * (xi0 - 3.0 * I * u * xi1);
+        double d2 = 2.0 * den * den;
+        for (int j = 0; j < 3; j++) {
+            b[0][j] = (2.0 * u * r1[j] + (3.0 * u2 - w2) * r2[j] - 2.0 * (15.0 * u2 + w2) * f[j]) / d2;
+            b[1][j] = (r1[j] - 3.0 * u * r2[j] - 24.0 * u * f[j]) / d2;
+        }
+    }
+    if (neg) {
+        for (int j = 0; j < 3; j++) {
+            double sg = (j % 2) ? -1.0 : 1.0;
+            f[j] = sg * conj(f[j]);
+            if (b) {
+                b[0][j] = sg * conj(b[0][j]);
+                b[1][j] = -sg * conj(b[1][j]);
+            }
+        }
+    }
+}
+
+/* c1 below which exp(iQ) uses 1 + iQ - Q^2/2 (the formulas divide by 9u^2-w^2 ~ 2 c1) */
+#define CH_EXP_TINY 1e-20
+/* c1 below which C star dexp(Q)/dQ uses its power series (the b_ij formulas cancel) */
+#define CH_DER_SERIES 1e-2
+#define CH_DER_TERMS 12
+
+/* exp(iH) for Hermitian traceless H */
+static m3 expi_herm(m3 H) {
+    m3 H2 = mul(H, H);
+    double c1 = 0.5 * creal(trace(H2));
+    m3 r;
+    if (c1 < CH_EXP_TINY) {
+        r = add(m3_id(), add(scale(I, H), scale(-0.5, H2)));
+        return r;
+    }
+    cplx f[3];
+    ch_coeffs(H, f, NULL);
+    r = add(scale(f[0], m3_id()), add(scale(f[1], H), scale(f[2], H2)));
+    return r;
+}
+
+/* exptU! for TA fields (TA_gaugefields_4D_serial.jl:760-1077): exp(t sum_a c_a i lambda_a/2).
+ * The reference diagonalizes H; the same matrix exponential is formed here with the
+ * Cayley-Hamilton coefficients of calc_coefficients_Q. */
+static m3 exp_ta(const double *c, double t) { return expi_herm(herm_from_coeffs(c, t)); }
+
+/* CdexpQdQ! (stout_fast.jl:888-946) with construct_B1B2!/construct_trCB1B2/
+ * construct_CdeQdQ_3! (stout_fast.jl:1031-1081): L such that d tr(C e^Q) = tr(L dQ) for
+ * anti-Hermitian traceless Q.  For c1 < CH_DER_SERIES the same derivative is summed from
+ * the series  L = sum_{n>=1} 1/n! sum_{k<n} Q^(n-1-k) C Q^k  (the reference instead
+ * leaves the output unwritten for tr Q^2 <= 1e-18). */
+static m3 cdexpqdq(m3 C, m3 Qa) {
+    m3 Qh = scale(-I, Qa); /* Q / i */
+    m3 Qh2 = mul(Qh, Qh);
+    double c1 = 0.5 * creal(trace(Qh2));
+    if (c1 < CH_DER_SERIES) {
+        m3 L = m3_zero(), Ln = C, CXn = C;
+        double fact = 1.0;
+        for (int n = 1; n <= CH_DER_TERMS; n++) {
+            fact *= n;
+            L = add(L, scale(1.0 / fact, Ln));
+            CXn = mul(CXn, Qa);
+            Ln = add(mul(Qa, Ln), CXn);
+        }
+        return L;
+    }
+    cplx f[3], b[2][3];
+    ch_coeffs(Qh, f, b);
+    m3 B1 = add(scale(b[0][0], m3_id()), add(scale(b[0][1], Qh), scale(b[0][2], Qh2)));
+    m3 B2 = add(scale(b[1][0], m3_id()), add(scale(b[1][1], Qh), scale(b[1][2], Qh2)));
+    cplx t1 = trace(mul(C, B1)), t2 = trace(mul(C, B2));
+    m3 L = add(scale(t1, Qh), scale(t2, Qh2));
+    L = add(L, scale(f[1], C));
+    L = add(L, scale(f[2], add(mul(Qh, C), mul(C, Qh))));
+    return scale(-I, L); /* divide by i */
+}
+
+/* public single-matrix entry points (used by the tests to pin the primitives) */
+void gfo_ta_coeffs(const double *m, double *c8) {
+    for (int a = 0; a < 8; a++) c8[a] = 0.0;
+    ta_add(ld(m), 1.0, c8);
+}
+void gfo_ta_matrix(const double *m, double *out) { st(out, ta_mat(ld(m))); }
+void gfo_exp_ta(const double *c8, double t, double *out) { st(out, exp_ta(c8, t)); }
+void gfo_exp_antiherm(const double *q, double *out) { st(out, expi_herm(scale(-I, ld(q)))); }
+void gfo_cdexpqdq(const double *c, const double *q, double *out) { st(out, cdexpqdq(ld(c), ld(q))); }
+
+/* ------------------------------------------------------------------------- */
+/* staples                                                                    */
+/* ------------------------------------------------------------------------- */
+/* construct_staple! (AbstractGaugefields.jl:2873-2923):
+ * V_mu(x) = sum_{nu != mu} U_nu(x) U_mu(x+nu) U_nu(x+mu)^dag */
+static m3 staple_upper(const lat *L, const double *U, int mu, long s) {
+    m3 V = m3_zero();
+    for (int nu = 0; nu < 4; nu++) {
+        if (nu == mu) continue;
+        m3 U1U2 = mul(LINK(U, L, nu, s), LINK(U, L, mu, UP(L, nu, s)));
+        V = add(V, mul_nd(U1U2, LINK(U, L, nu, UP(L, mu, s))));
+    }
+    return V;
+}
+
+/* loops_staple_prime / make_plaq_staple_prime (autostaples/wilsonloops.jl:468-485),
+ * evaluated by construct_double_staple! (AbstractGaugefields.jl:2856-2871); also the
+ * stout C_mu / rho of calc_C! (stout_fast.jl:603-624) for the plaquette loop set:
+ * V_mu(x) = sum_{nu != mu} [U_nu(x) U_mu(x+nu) U_nu(x+mu)^dag
+ *                          + U_nu(x-nu)^dag U_mu(x-nu) U_nu(x-nu+mu)] */
+static m3 staple_full(const lat *L, const double *U, int mu, long s) {
+    m3 V = m3_zero();
+    for (int nu = 0; nu < 4; nu++) {
+        if (nu == mu) continue;
+        m3 a = mul(LINK(U, L, nu, s), LINK(U, L, mu, UP(L, nu, s)));
+        V = add(V, mul_nd(a, LINK(U, L, nu, UP(L, mu, s))));
+        long sm = DN(L, nu, s);
+        m3 b = mul_dn(LINK(U, L, nu, sm), LINK(U, L, mu, sm));
+        V = add(V, mul(b, LINK(U, L, nu, UP(L, mu, sm))));
+    }
+    return V;
+}
+
+/* ------------------------------------------------------------------------- */
+/* plaquette, action                                                          */
+/* ------------------------------------------------------------------------- */
+static double plaq_site(const lat *L, long s, const void *ctx) {
+    const double *U = (const double *)ctx;
+    double acc = 0.0;
+    for (int mu = 0; mu < 4; mu++) {
+        m3 V = staple_upper(L, U, mu, s);
+        acc += creal(trace(mul_nd(LINK(U, L, mu, s), V)));
+    }
+    return acc;
+}
+
+/* calculate_Plaquette (AbstractGaugefields.jl:2684-2699): real(0.5 sum_mu tr(U_mu V_mu^dag))
+ * = sum_{x, mu<nu} Re tr P_munu(x).  measure_plaquette (API.jl:395-404) divides by 6*V*3. */
+double gfo_plaquette_sum(const int *dims, const double *U) {
+    lat L = lat_make(dims);
+    double v = 0.5 * lattice_sum(&L, plaq_site, U);
+    lat_free(&L);
+    return v;
+}
+
+/* md_potential(::GaugeAction) (molecular_dynamics.jl:247-249) for push!(action, coeff,
+ * plaquette ∪ plaquette^dag): evaluate_GaugeAction (GaugeActions.jl:132-142) sums coeff*tr
+ * over 12 oriented loops per site = 2 coeff sum Re tr P; potential = -that / NC.  The same
+ * normalization is restated in test/enzyme_md.jl:17-40. */
+double gfo_potential(const int *dims, const double *U, double coeff) {
+    return -2.0 * coeff * gfo_plaquette_sum(dims, U) / 3.0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* derivative, force, kick, exp update                                        */
+/* ------------------------------------------------------------------------- */
+/* calc_dSdUμ! (GaugeActions.jl:95-123) with the six plaquette staples per direction that
+ * Wilsonloop.make_staple yields for plaquette ∪ plaquette^dag (GaugeActions.jl:48-62):
+ * coeff * sum_{nu != mu} [U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+ *                        + U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)]
+ * so that tr(U_mu(x) dSdU_mu(x)) closes the plaquettes through U_mu(x). */
+static m3 dsdu_site(const lat *L, const double *U, int mu, long s) {
+    m3 D = m3_zero();
+    for (int nu = 0; nu < 4; nu++) {
+        if (nu == mu) continue;
+        m3 a = mul_nd(LINK(U, L, nu, UP(L, mu, s)), LINK(U, L, mu, UP(L, nu, s)));
+        D = add(D, mul_nd(a, LINK(U, L, nu, s)));
+        long sm = DN(L, nu, s);
+        m3 b = mul(LINK(U, L, mu, sm), LINK(U, L, nu, UP(L, mu, sm)));
+        D = add(D, mul_dn(b, LINK(U, L, nu, sm)));
+    }
+    return D;
+}
+
+void gfo_dsdu(const int *dims, const double *U, double coeff, double *out) {
+    lat L = lat_make(dims);
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < L.V; s++)
+        for (int mu = 0; mu < 4; mu++) st(LINKP(out, &L, mu, s), scale(coeff, dsdu_site(&L, U, mu, s)));
+    lat_free(&L);
+}
+
+/* md_force!(::GaugeAction) (molecular_dynamics.jl:251-267):
+ * F_mu = clear; Traceless_antihermitian_add!(F_mu, -1/NC, U_mu * dSdU_mu) */
+static void force_into(const lat *L, const double *U, double coeff, double *F) {
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < L->V; s++)
+        for (int mu = 0; mu < 4; mu++) {
+            double *c = F + 8 * ((long)mu * L->V + s);
+            for (int a = 0; a < 8; a++) c[a] = 0.0;
+            m3 D = scale(coeff, dsdu_site(L, U, mu, s));
+            ta_add(mul(LINK(U, L, mu, s), D), -1.0 / 3.0, c);
+        }
+}
+void gfo_force(const int *dims, const double *U, double coeff, double *F) {
+    lat L = lat_make(dims);
+    force_into(&L, U, coeff, F);
+    lat_free(&L);
+}
+
+/* update_momenta! kick (molecular_dynamics.jl:539-551): add_U!(P, eps, F)
+ * (TA_gaugefields_4D_serial.jl:150-171) */
+static void kick(long n8, double *P, const double *F, double eps) {
+    for (long i = 0; i < n8; i++) P[i] += eps * F[i];
+}
+void gfo_kick(const int *dims, double *P, const double *F, double eps) {
+    long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
+    kick(32 * V, P, F, eps);
+}
+
+/* update_gaugefields! (molecular_dynamics.jl:513-531): U_mu <- exp(eps P_mu) U_mu */
+static void exp_update(const lat *L, double *U, const double *P, double eps) {
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < L->V; s++)
+        for (int mu = 0; mu < 4; mu++) {
+            m3 E = exp_ta(P + 8 * ((long)mu * L->V + s), eps);
+            st(LINKP(U, L, mu, s), mul(E, LINK(U, L, mu, s)));
+        }
+}
+void gfo_exp_update(const int *dims, double *U, const double *P, double eps) {
+    lat L = lat_make(dims);
+    exp_update(&L, U, P, eps);
+    lat_free(&L);
+}
+
+/* p*p (TA_gaugefields_4D_serial.jl:106-129): sum of squares of all coefficients;
+ * md_hamiltonian uses real(p*p)/2 (molecular_dynamics.jl:494-505). */
+double gfo_momenta_dot(const int *dims, const double *P) {
+    long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
+    double s = 0.0;
+    for (long i = 0; i < 32 * V; i++) s += P[i] * P[i];
+    return s;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Wilson flow                                                                */
+/* ------------------------------------------------------------------------- */
+/* add_force!(F, U; plaqonly=true) (AbstractGaugefields.jl:2717-2762) into cleared F:
+ * F_mu = TA(U_mu V_mu^dag), V = loops_staple_prime sum. */
+static void flow_force(const lat *L, const double *U, double *F) {
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < L->V; s++)
+        for (int mu = 0; mu < 4; mu++) {
+            double *c = F + 8 * ((long)mu * L->V + s);
+            for (int a = 0; a < 8; a++) c[a] = 0.0;
+            m3 V = staple_full(L, U, mu, s);
+            ta_add(mul_nd(LINK(U, L, mu, s), V), 1.0, c);
+        }
+}
+void gfo_flow_force(const int *dims, const double *U, double *F) {
+    lat L = lat_make(dims);
+    flow_force(&L, U, F);
+    lat_free(&L);
+}
+
+/* exp_aF_U! (AbstractGaugefields.jl:2810-2841): W_mu = exp(a F_mu) U_mu */
+static void exp_aF_U(const lat *L, double *W, double a, const double *F, const double *U) {
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < L->V; s++)
+        for (int mu = 0; mu < 4; mu++) {
+            m3 E = exp_ta(F + 8 * ((long)mu * L->V + s), a);
+            st(LINKP(W, L, mu, s), mul(E, LINK(U, L, mu, s)));
+        }
+}
+
+/* Plaquette energy density (the reference has no built-in E(t); Luscher, JHEP 08 (2010)
+ * 071): E = (2/V) sum_{x, mu<nu} Re tr(1 - P_munu(x)) = 36 (1 - plaquette/(18 V)). */
+double gfo_energy_density(const int *dims, const double *U) {
+    double V = (double)dims[0] * dims[1] * dims[2] * dims[3];
+    return 2.0 * (18.0 - gfo_plaquette_sum(dims, U) / V);
+}
+
+/* flow!(U, ::Gradientflow) (gradientflow.jl:171-238): Luscher RK3,
+ * W1 = exp(-eps/4 F0) U;  W2 = exp(-8eps/9 F1 + 17eps/36 F0) W1;
+ * U  = exp(-3eps/4 F2 + 8eps/9 F1 - 17eps/36 F0) W2.
+ * E_out[k] = energy density after step (k+1)*every (when every > 0). */
+void gfo_flow(const int *dims, double *U, double eps, int nsteps, int every, double *E_out) {
+    lat L = lat_make(dims);
+    long n18 = 18 * 4 * L.V, n8 = 8 * 4 * L.V;
+    double *W1 = (double *)malloc(sizeof(double) * n18);
+    double *W2 = (double *)malloc(sizeof(double) * n18);
+    double *F0 = (double *)malloc(sizeof(double) * n8);
+    double *F1 = (double *)malloc(sizeof(double) * n8);
+    double *F2 = (double *)malloc(sizeof(double) * n8);
+    double *Ft = (double *)malloc(sizeof(double) * n8);
+    int k = 0;
+    for (int step = 1; step <= nsteps; step++) {
+        flow_force(&L, U, F0);
+        exp_aF_U(&L, W1, -eps / 4.0, F0, U);
+        flow_force(&L, W1, F1);
+        for (long i = 0; i < n8; i++) Ft[i] = 0.0;
+        kick(n8, Ft, F1, -(8.0 * eps / 9.0));
+        kick(n8, Ft, F0, 17.0 * eps / 36.0);
+        exp_aF_U(&L, W2, 1.0, Ft, W1);
+        flow_force(&L, W2, F2);
+        for (long i = 0; i < n8; i++) Ft[i] = 0.0;
+        kick(n8, Ft, F2, -(3.0 * eps / 4.0));
+        kick(n8, Ft, F1, 8.0 * eps / 9.0);
+        kick(n8, Ft, F0, -(17.0 * eps / 36.0));
+        exp_aF_U(&L, U, 1.0, Ft, W2);
+        if (every > 0 && step % every == 0 && E_out) E_out[k++] = gfo_energy_density(dims, U);
+    }
+    free(W1);
+    free(W2);
+    free(F0);
+    free(F1);
+    free(F2);
+    free(Ft);
+    lat_free(&L);
+}
+
+/* ------------------------------------------------------------------------- */
+/* stout smearing                                                             */
+/* ------------------------------------------------------------------------- */
+/* forward! (stout_fast.jl:250-274) with calc_C! (stout_fast.jl:603-624) for the plaquette
+ * loop set: C_mu = rho V_mu (staple_full), Q_mu = TA(C_mu U_mu^dag),
+ * Uout_mu = exp(Q_mu) U_mu.  C (optional) receives C_mu. */
+static void stout_forward(const lat *L, const double *Uin, double rho, double *Uout, double *C) {
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < L->V; s++)
+        for (int mu = 0; mu < 4; mu++) {
+            m3 U = LINK(Uin, L, mu, s);
+            m3 Cm = scale(rho, staple_full(L, Uin, mu, s));
+            m3 Q = ta_mat(mul_nd(Cm, U));
+            m3 E = expi_herm(scale(-I, Q));
+            st(LINKP(Uout, L, mu, s), mul(E, U));
+            if (C) st(LINKP(C, L, mu, s), Cm);
+        }
+}
+void gfo_stout_forward(const int *dims, const double *Uin, double rho, double *Uout, double *C) {
+    lat L = lat_make(dims);
+    stout_forward(&L, Uin, rho, Uout, C);
+    lat_free(&L);
+}
+
+/* backward_dSdUαUβρ_add! (stout_fast.jl:317-407) for one plaquette stout level, with the
+ * convention of the reference's matrix derivatives (dS = 2 Re tr(D dU)):
+ *   Din_mu  = Dout_mu e^{Q_mu}                                  calc_dSdu1!      :629-631
+ *   L       = CdexpQdQ(U_mu Dout_mu, Q_mu)                       calc_dSdQ!       :636-675
+ *   Lam     = TA(L)                                              calc_dSdΩ!       :682-684
+ *   G_mu    = U_mu^dag Lam                                       calc_dSdC!       :686-688
+ *   Din_mu += (Lam C_mu)^dag                                     calc_dSdUdag!    :690-694
+ *   Din_nu += rho sum_j B_j G_mu(x - pos_j) A_j  (and the C^dag terms)
+ *                                                   calc_dSdUν_fromdSCμ_add! :712-785
+ * The last line's terms are those of Wilsonloop.derive_U applied to the six plaquette
+ * staples of C_mu (stout_dataset.jl:10-105); written out for output direction a at site y
+ * and staple plane b != a they are
+ *   (i)   U_b(y-b+a)^dag G_a(y-b) U_b(y-b)         (ii)  U_b(y+a) G_a(y+b) U_b(y)^dag
+ *   (iii) U_b(y+a) U_a(y+b)^dag G_b(y)             (iv)  U_b(y-b+a)^dag U_a(y-b)^dag G_b(y-b)^dag
+ *   (v)   G_b(y+a)^dag U_a(y+b)^dag U_b(y)^dag     (vi)  G_b(y+a-b) U_a(y-b)^dag U_b(y-b)      */
+static void stout_backward(const lat *L, const double *Uin, double rho, const double *Dout, double *Din) {
+    long n18 = 18 * 4 * L->V;
+    double *G = (double *)malloc(sizeof(double) * n18);
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < L->V; s++)
+        for (int mu = 0; mu < 4; mu++) {
+            m3 U = LINK(Uin, L, mu, s);
+            m3 Cm = scale(rho, staple_full(L, Uin, mu, s));
+            m3 Q = ta_mat(mul_nd(Cm, U));
+            m3 E = expi_herm(scale(-I, Q));
+            m3 Do = LINK(Dout, L, mu, s);
+            m3 D = mul(Do, E);
+            m3 Lm = ta_mat(cdexpqdq(mul(U, Do), Q));
+            st(LINKP(G, L, mu, s), mul_dn(U, Lm));
+            D = add(D, dag(mul(Lm, Cm)));
+            st(LINKP(Din, L, mu, s), D);
+        }
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long y = 0; y < L->V; y++)
+        for (int a = 0; a < 4; a++) {
+            m3 acc = m3_zero();
+            for (int b = 0; b < 4; b++) {
+                if (b == a) continue;
+                long ymb = DN(L, b, y), ypa = UP(L, a, y), ypb = UP(L, b, y);
+                long ymbpa = UP(L, a, ymb);
+                /* (i) */
+                acc = add(acc, mul(mul_dn(LINK(Uin, L, b, ymbpa), LINK(G, L, a, ymb)), LINK(Uin, L, b, ymb)));
+                /* (ii) */
+                acc = add(acc, mul_nd(mul(LINK(Uin, L, b, ypa), LINK(G, L, a, ypb)), LINK(Uin, L, b, y)));
+                /* (iii) */
+                acc = add(acc, mul(mul_nd(LINK(Uin, L, b, ypa), LINK(Uin, L, a, ypb)), LINK(G, L, b, y)));
+                /* (iv) */
+                acc = add(acc, mul_nd(mul_dn(LINK(Uin, L, b, ymbpa), dag(LINK(Uin, L, a, ymb))), LINK(G, L, b, ymb)));
+                /* (v) */
+                acc = add(acc, mul_nd(mul_dn(LINK(G, L, b, ypa), dag(LINK(Uin, L, a, ypb))), LINK(Uin, L, b, y)));
+                /* (vi) */
+                long ypamb = DN(L, b, ypa);
+                acc = add(acc, mul(mul_nd(LINK(G, L, b, ypamb), LINK(Uin, L, a, ymb)), LINK(Uin, L, b, ymb)));
+            }
+            double *p = LINKP(Din, L, a, y);
+            st(p, add(ld(p), scale(rho, acc)));
+        }
+    free(G);
+}
+void gfo_stout_backward(const int *dims, const double *Uin, double rho, const double *Dout, double *Din) {
+    lat L = lat_make(dims);
+    stout_backward(&L, Uin, rho, Dout, Din);
+    lat_free(&L);
+}
+
+/* calc_smearedU / apply_neuralnet! (Abstractsmearing.jl:237-272, 468-485): nlev stout
+ * layers applied in sequence. */
+void gfo_stout_smear(const int *dims, const double *U, int nlev, double rho, double *Uout) {
+    lat L = lat_make(dims);
+    long n18 = 18 * 4 * L.V;
+    double *a = (double *)malloc(sizeof(double) * n18);
+    memcpy(a, U, sizeof(double) * n18);
+    for (int l = 0; l < nlev; l++) {
+        stout_forward(&L, a, rho, Uout, NULL);
+        memcpy(a, Uout, sizeof(double) * n18);
+    }
+    if (nlev == 0) memcpy(Uout, U, sizeof(double) * n18);
+    free(a);
+    lat_free(&L);
+}
+
+double gfo_stout_potential(const int *dims, const double *U, int nlev, double rho, double coeff) {
+    long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
+    double *S = (double *)malloc(sizeof(double) * 72 * V);
+    gfo_stout_smear(dims, U, nlev, rho, S);
+    double v = gfo_potential(dims, S, coeff);
+    free(S);
+    return v;
+}
+
+/* Stout-smeared force (test/HMCstout_test_nowing.jl:99-118): smear (storing every level,
+ * Abstractsmearing.jl:280-314), calc_dSdUμ! on the smeared links, back_prop through the
+ * levels (Abstractsmearing.jl:352-411), F_mu = -(1/NC) TA(U_mu dSdU_bare_mu). */
+static void stout_force(const lat *L, const double *U, int nlev, double rho, double coeff, double *F) {
+    long n18 = 18 * 4 * L->V;
+    double *lev = (double *)malloc(sizeof(double) * n18 * (nlev + 1));
+    double *D = (double *)malloc(sizeof(double) * n18);
+    double *D2 = (double *)malloc(sizeof(double) * n18);
+    memcpy(lev, U, sizeof(double) * n18);
+    for (int l = 0; l < nlev; l++) stout_forward(L, lev + n18 * l, rho, lev + n18 * (l + 1), NULL);
+    const double *Us = lev + n18 * nlev;
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < L->V; s++)
+        for (int mu = 0; mu < 4; mu++) st(LINKP(D, L, mu, s), scale(coeff, dsdu_site(L, Us, mu, s)));
+    for (int l = nlev - 1; l >= 0; l--) {
+        stout_backward(L, lev + n18 * l, rho, D, D2);
+        double *t = D;
+        D = D2;
+        D2 = t;
+    }
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < L->V; s++)
+        for (int mu = 0; mu < 4; mu++) {
+            double *c = F + 8 * ((long)mu * L->V + s);
+            for (int a = 0; a < 8; a++) c[a] = 0.0;
+            ta_add(mul(LINK(U, L, mu, s), LINK(D, L, mu, s)), -1.0 / 3.0, c);
+        }
+    free(lev);
+    free(D);
+    free(D2);
+}
+void gfo_stout_force(const int *dims, const double *U, int nlev, double rho, double coeff, double *F) {
+    lat L = lat_make(dims);
+    stout_force(&L, U, nlev, rho, coeff, F);
+    lat_free(&L);
+}
+
+/* ------------------------------------------------------------------------- */
+/* molecular dynamics                                                         */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    const lat *L;
+    const int *dims;
+    double coeff, rho;
+    int nlev;
+    double *F;
+} md_ctx;
+
+/* update_momenta! (molecular_dynamics.jl:539-551) */
+static void md_P(md_ctx *m, double *U, double *P, double eps) {
+    if (m->nlev > 0)
+        stout_force(m->L, U, m->nlev, m->rho, m->coeff, m->F);
+    else
+        force_into(m->L, U, m->coeff, m->F);
+    kick(32 * m->L->V, P, m->F, eps);
+}
+/* update_gaugefields! (molecular_dynamics.jl:513-531) */
+static void md_Q(md_ctx *m, double *U, double *P, double eps) { exp_update(m->L, U, P, eps); }
+
+static double md_H(md_ctx *m, const double *U, const double *P) {
+    double V = m->nlev > 0 ? gfo_stout_potential(m->dims, U, m->nlev, m->rho, m->coeff)
+                           : gfo_potential(m->dims, U, m->coeff);
+    return V + 0.5 * gfo_momenta_dot(m->dims, P);
+}
+
+/* md_trajectory! (molecular_dynamics.jl:712-730) with md_step! QPQ / PQP
+ * (molecular_dynamics.jl:604-616) and the Omelyan 2MN user integrator of
+ * docs/src/howtoimplement.md:94-112.  H = {H0, H1, H1 - H0}. */
+void gfo_md_trajectory(const int *dims, double *U, double *P, double coeff, int nlev, double rho,
+                       int integrator, double lambda, int steps, double tau, double *H) {
+    lat L = lat_make(dims);
+    md_ctx m = {&L, dims, coeff, rho, nlev, (double *)malloc(sizeof(double) * 32 * L.V)};
+    double h0 = md_H(&m, U, P);
+    double eps = tau / steps;
+    for (int k = 0; k < steps; k++) {
+        if (integrator == 0) {
+            md_Q(&m, U, P, eps / 2);
+            md_P(&m, U, P, eps);
+            md_Q(&m, U, P, eps / 2);
+        } else if (integrator == 1) {
+            md_P(&m, U, P, eps / 2);
+            md_Q(&m, U, P, eps);
+            md_P(&m, U, P, eps / 2);
+        } else {
+            md_P(&m, U, P, lambda * eps);
+            md_Q(&m, U, P, 0.5 * eps);
+            md_P(&m, U, P, (1.0 - 2.0 * lambda) * eps);
+            md_Q(&m, U, P, 0.5 * eps);
+            md_P(&m, U, P, lambda * eps);
+        }
+    }
+    double h1 = md_H(&m, U, P);
+    H[0] = h0;
+    H[1] = h1;
+    H[2] = h1 - h0;
+    free(m.F);
+    lat_free(&L);
+}

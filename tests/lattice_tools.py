@@ -1,0 +1,132 @@
+"""Independent numpy restatements used by the tests as a second check on the oracle.
+
+Whole-lattice np.roll formulations (not the oracle's site loops), the reference's
+instanton constructor, gauge transformations and finite differences.
+"""
+import os
+import sys
+
+import numpy as np
+from scipy.linalg import expm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import __graft_entry__  # noqa: E402
+
+gen = __graft_entry__.load_package().generators
+LAMBDA = gen.LAMBDA
+T_GEN = LAMBDA / 2
+
+
+def field(U, dims):
+    """(4, V, 9) reference layout -> (4, NT, NZ, NY, NX, 3, 3)."""
+    NX, NY, NZ, NT = dims
+    return gen.links_to_matrices(U).reshape(4, NT, NZ, NY, NX, 3, 3)
+
+
+def unfield(M):
+    return gen.matrices_to_links(M.reshape(M.shape[0], -1, 3, 3))
+
+
+_AX = {0: 3, 1: 2, 2: 1, 3: 0}
+
+
+def sh(A, mu, d=1):
+    """A(x + d*mu) for A of shape (NT, NZ, NY, NX, ...)."""
+    return np.roll(A, -d, axis=_AX[mu])
+
+
+def dg(A):
+    return np.conj(np.swapaxes(A, -1, -2))
+
+
+def plaquette_field_sum(U, dims):
+    """sum_{x, mu<nu} Re tr U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag (any NC)."""
+    M = field(U, dims) if U.ndim == 3 else U
+    tot = 0.0
+    for mu in range(4):
+        for nu in range(mu + 1, 4):
+            P = M[mu] @ sh(M[nu], mu) @ dg(sh(M[mu], nu)) @ dg(M[nu])
+            tot += np.trace(P, axis1=-2, axis2=-1).real.sum()
+    return tot
+
+
+def ta_matrix(M):
+    A = 0.5 * (M - dg(M))
+    tr = np.trace(A, axis1=-2, axis2=-1)
+    return A - (tr / 3.0)[..., None, None] * np.eye(3)
+
+
+def coeffs_to_matrix(c):
+    """sum_a c_a i lambda_a / 2 for c of shape (..., 8)."""
+    return np.einsum("...a,aij->...ij", c, 1j * T_GEN)
+
+
+def su2_instanton_link(mu, ix, iy, iz, it, L):
+    """_su2_instanton_link (AbstractGaugefields.jl:1126-1170), default centre/radius, sign +1."""
+    center = [L[k] / 2 + 0.5 for k in range(4)]
+    radius = L[0] // 2
+    s1 = np.array([[0, 1], [1, 0]], complex)
+    s2 = np.array([[0, -1j], [1j, 0]], complex)
+    s3 = np.array([[1, 0], [0, -1]], complex)
+    En = np.eye(2, dtype=complex)
+    ss = (1j * s1, 1j * s2, 1j * s3, En)
+    sd = (-1j * s1, -1j * s2, -1j * s3, En)
+    nv = np.array([ix - 1 - center[0], iy - 1 - center[1], iz - 1 - center[2], it - 1 - center[3]])
+    n2 = float(nv @ nv)
+    tau = np.zeros((2, 2), complex)
+    for nu in range(4):
+        tau += (sd[mu] @ ss[nu] - sd[nu] @ ss[mu]) * nv[nu]
+    return expm(1j * tau * 0.5 * (1.0 / n2) * (1j * radius ** 2 / (n2 + radius ** 2)))
+
+
+def instanton_links(dims, nc=3, block=(0, 1)):
+    """Oneinstanton_4D_wing (gaugefields_4D_wing.jl:388-431), embedded in SU(nc) at `block`."""
+    NX, NY, NZ, NT = dims
+    V = NX * NY * NZ * NT
+    M = np.zeros((4, V, nc, nc), complex)
+    M[..., range(nc), range(nc)] = 1.0
+    i, j = block
+    for it in range(1, NT + 1):
+        for iz in range(1, NZ + 1):
+            for iy in range(1, NY + 1):
+                for ix in range(1, NX + 1):
+                    s = (ix - 1) + NX * ((iy - 1) + NY * ((iz - 1) + NZ * (it - 1)))
+                    for mu in range(4):
+                        u2 = su2_instanton_link(mu, ix, iy, iz, it, dims)
+                        M[mu, s, i, i] = u2[0, 0]
+                        M[mu, s, i, j] = u2[0, 1]
+                        M[mu, s, j, i] = u2[1, 0]
+                        M[mu, s, j, j] = u2[1, 1]
+    return M
+
+
+def random_su3(n, rng):
+    g = rng.standard_normal((n, 3, 3)) + 1j * rng.standard_normal((n, 3, 3))
+    q, r = np.linalg.qr(g)
+    d = np.diagonal(r, axis1=-2, axis2=-1)
+    q = q * (d / np.abs(d))[:, None, :]
+    det = np.linalg.det(q)
+    return q / (det ** (1.0 / 3.0))[:, None, None]
+
+
+def gauge_transform(U, dims, g):
+    """U_mu(x) -> g(x) U_mu(x) g(x+mu)^dag; g has shape (V, 3, 3)."""
+    NX, NY, NZ, NT = dims
+    M = field(U, dims)
+    G = g.reshape(NT, NZ, NY, NX, 3, 3)
+    out = np.empty_like(M)
+    for mu in range(4):
+        out[mu] = G @ M[mu] @ dg(sh(G, mu))
+    return unfield(out)
+
+
+def perturb_link(U, mu, s, a, h):
+    """copy of U with U_mu(s) -> exp(h i T_a) U_mu(s)."""
+    V = U.copy()
+    M = gen.links_to_matrices(V)
+    E = expm(h * 1j * T_GEN[a])
+    M[mu, s] = E @ M[mu, s]
+    return gen.matrices_to_links(M)

@@ -1,0 +1,364 @@
+"""Pins of the CPU oracle (oracle/gf_oracle.c) -- CPU only.
+
+Each oracle function is checked against something it does not itself restate: the
+reference's golden values, closed forms, scipy routines, a numpy whole-lattice brute
+force, finite differences of the potential, and gauge symmetry.  The layout of each test
+names the reference file:line whose behaviour it pins.
+"""
+import numpy as np
+import pytest
+from scipy.linalg import expm, expm_frechet
+
+import lattice_tools as lt
+
+gen = lt.gen
+D4 = (4, 4, 4, 4)
+BETA = 6.0
+COEFF = BETA / 2  # push!(action, beta/2, plaquette ∪ plaquette^dag)
+
+
+def hot(dims, seed):
+    return gen.hot_start_links(dims, seed)
+
+
+def alg_links(oracle, dims, sigma, seed):
+    U = gen.cold_links(dims)
+    oracle.exp_update(dims, U, gen.algebra_coefficients(dims, sigma, seed), 1.0)
+    return U
+
+
+# ---------------------------------------------------------------- plaquette
+
+
+def test_cold_plaquette_is_one(oracle):
+    # test/init.jl cold testsets; test/high_level_api.jl:48-49
+    for dims in [D4, (2, 2, 2, 2), (4, 2, 3, 2)]:
+        U = gen.cold_links(dims)
+        V = np.prod(dims)
+        assert oracle.plaquette_sum(dims, U) / (6 * V * 3) == 1.0
+
+
+def test_instanton_golden(oracle):
+    # test/init.jl:351-371 calls Oneinstanton(NC, NX, NY, NZ, NT, Nwing) against the
+    # signature Oneinstanton(NC, NDW, NN...) (AbstractGaugefields.jl:998), so its lattice
+    # is 4x4x4x1 (halo 4): SU(2) plaquette 0.9796864531099871.  SU(3) embedding gives
+    # (2p + 1)/3 (test/sun_embedded_instanton.jl:145).
+    p2_ref = 0.9796864531099871
+    dims = (4, 4, 4, 1)
+    V = 64
+    M2 = lt.instanton_links(dims, nc=2)
+    p2 = lt.plaquette_field_sum(M2.reshape(4, 1, 4, 4, 4, 2, 2), dims) / (6 * V * 2)
+    assert abs(p2 - p2_ref) / p2_ref < 1e-14
+    for block in [(0, 1), (0, 2), (1, 2)]:
+        U = gen.matrices_to_links(lt.instanton_links(dims, nc=3, block=block))
+        p3 = oracle.plaquette_sum(dims, U) / (6 * V * 3)
+        assert abs(p3 - (2 * p2_ref + 1) / 3) < 1e-14
+
+
+@pytest.mark.parametrize("dims", [D4, (2, 2, 2, 2), (4, 2, 3, 2), (2, 4, 2, 6)])
+def test_plaquette_vs_numpy_bruteforce(oracle, dims):
+    U = hot(dims, 7)
+    ref = lt.plaquette_field_sum(U, dims)
+    assert abs(oracle.plaquette_sum(dims, U) - ref) < 1e-12 * abs(ref) + 1e-12
+
+
+def test_plaquette_gauge_invariant(oracle):
+    rng = np.random.default_rng(3)
+    U = alg_links(oracle, D4, 0.3, 5)
+    g = lt.random_su3(256, rng)
+    Ug = lt.gauge_transform(U, D4, g)
+    a, b = oracle.plaquette_sum(D4, U), oracle.plaquette_sum(D4, Ug)
+    assert abs(a - b) < 1e-13 * abs(a)
+
+
+def test_potential_normalization(oracle):
+    # md_potential = -real(evaluate_GaugeAction)/NC, evaluate = 2*coeff*sum Re tr P
+    # (molecular_dynamics.jl:247-249, test/enzyme_md.jl:17-40)
+    U = hot(D4, 9)
+    M = lt.field(U, D4)
+    val = 0.0
+    for mu in range(4):
+        for nu in range(mu + 1, 4):
+            P = M[mu] @ lt.sh(M[nu], mu) @ lt.dg(lt.sh(M[mu], nu)) @ lt.dg(M[nu])
+            Pr = M[nu] @ lt.sh(M[mu], nu) @ lt.dg(lt.sh(M[nu], mu)) @ lt.dg(M[mu])
+            val += np.trace(P, axis1=-2, axis2=-1).real.sum() + np.trace(Pr, axis1=-2, axis2=-1).real.sum()
+    assert abs(oracle.potential(D4, U, 2.75) - (-2.75 * val / 3)) < 1e-12 * abs(val)
+
+
+# ---------------------------------------------------------------- single-matrix primitives
+
+
+def test_ta_projection_closed_form(oracle):
+    # Traceless_antihermitian_add! (TA_gaugefields_4D_serial.jl:181-269): coefficients of
+    # TA(M) = (M - M^dag)/2 - tr(M - M^dag)/6 in the basis i lambda_a / 2
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        M = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
+        c = oracle.ta_coeffs(M)
+        assert np.abs(lt.coeffs_to_matrix(c) - lt.ta_matrix(M)).max() < 1e-14
+        assert np.abs(oracle.ta_matrix(M) - lt.ta_matrix(M)).max() < 1e-15
+
+
+@pytest.mark.parametrize("scale", [3.0, 1.0, 0.3, 0.05, 1e-3, 1e-7, 0.0])
+def test_exp_vs_scipy(oracle, scale):
+    rng = np.random.default_rng(int(scale * 1e6) + 1)
+    for trial in range(40):
+        c = scale * rng.standard_normal(8)
+        if trial % 2:
+            c = -c
+        t = 0.7
+        E = oracle.exp_ta(c, t)
+        ref = expm(t * lt.coeffs_to_matrix(c))
+        assert np.abs(E - ref).max() < 2e-15 * max(1.0, scale * 4)
+        assert np.abs(E.conj().T @ E - np.eye(3)).max() < 5e-15
+        assert abs(np.linalg.det(E) - 1) < 5e-15
+
+
+def test_exp_degenerate_spectra(oracle):
+    # two equal eigenvalues of either sign of det Q (9u^2 = w^2 without the c0 symmetry)
+    for c8 in [0.3, -0.3, 1.7, -1.7]:
+        c = np.zeros(8)
+        c[7] = c8
+        assert np.abs(oracle.exp_ta(c, 1.0) - expm(lt.coeffs_to_matrix(c))).max() < 2e-15
+    c = np.zeros(8)
+    c[2] = 0.4
+    assert np.abs(oracle.exp_ta(c, 1.0) - expm(lt.coeffs_to_matrix(c))).max() < 2e-15
+
+
+@pytest.mark.parametrize("scale", [2.0, 0.6, 0.3, 0.15, 0.05, 1e-3, 1e-6])
+def test_cdexpqdq_vs_expm_frechet(oracle, scale):
+    # CdexpQdQ! (stout_fast.jl:888-946): d tr(C e^Q) = tr(L dQ) along su(3)
+    rng = np.random.default_rng(int(scale * 1e4) + 2)
+    for trial in range(30):
+        c = scale * rng.standard_normal(8)
+        if trial % 2:
+            c = -c
+        Q = lt.coeffs_to_matrix(c)
+        C = rng.standard_normal((3, 3)) + 1j * rng.standard_normal((3, 3))
+        L = oracle.cdexpqdq(C, Q)
+        for a in range(8):
+            E = 1j * lt.T_GEN[a]
+            ref = np.trace(C @ expm_frechet(Q, E, compute_expm=False))
+            assert abs(np.trace(L @ E) - ref) < 3e-14 * max(1.0, np.abs(C).max())
+
+
+def test_exp_antiherm(oracle):
+    rng = np.random.default_rng(4)
+    Q = lt.coeffs_to_matrix(rng.standard_normal(8))
+    assert np.abs(oracle.exp_antiherm(Q) - expm(Q)).max() < 3e-15
+
+
+# ---------------------------------------------------------------- derivative and force
+
+
+def test_dsdu_closes_plaquettes(oracle):
+    # calc_dSdUμ! (GaugeActions.jl:95-123): sum_{x,mu} Re tr(U_mu dSdU_mu) = 4 coeff sum Re tr P
+    U = hot(D4, 21)
+    D = oracle.dsdu(D4, U, COEFF)
+    MU, MD = gen.links_to_matrices(U), gen.links_to_matrices(D)
+    tot = np.trace(MU @ MD, axis1=-2, axis2=-1).real.sum()
+    assert abs(tot - 4 * COEFF * oracle.plaquette_sum(D4, U)) < 1e-11 * abs(tot)
+
+
+def test_dsdu_vs_numpy(oracle):
+    dims = (2, 4, 2, 4)
+    U = hot(dims, 22)
+    M = lt.field(U, dims)
+    D = lt.field(oracle.dsdu(dims, U, 1.3), dims)
+    for mu in range(4):
+        ref = 0
+        for nu in range(4):
+            if nu == mu:
+                continue
+            ref = ref + lt.sh(M[nu], mu) @ lt.dg(lt.sh(M[mu], nu)) @ lt.dg(M[nu])
+            Mnm = lt.sh(M[nu], nu, -1)
+            ref = ref + lt.dg(lt.sh(Mnm, mu)) @ lt.dg(lt.sh(M[mu], nu, -1)) @ Mnm
+        assert np.abs(D[mu] - 1.3 * ref).max() < 1e-13
+
+
+def _fd_force(pot, U, mu, s, a, h=1e-5):
+    return -(pot(lt.perturb_link(U, mu, s, a, h)) - pot(lt.perturb_link(U, mu, s, a, -h))) / (2 * h)
+
+
+def test_force_vs_finite_difference(oracle):
+    # md_force! (molecular_dynamics.jl:251-267) is dp/dtau = -dV/domega_a along
+    # U -> exp(omega i T_a) U; central differences h=1e-5 on 64 random (link, generator)
+    U = alg_links(oracle, D4, 0.3, 31)
+    F = oracle.force(D4, U, COEFF)
+    rng = np.random.default_rng(5)
+    pot = lambda W: oracle.potential(D4, W, COEFF)  # noqa: E731
+    for _ in range(64):
+        mu, s, a = rng.integers(4), rng.integers(256), rng.integers(8)
+        fd = _fd_force(pot, U, mu, s, a)
+        assert abs(F[mu, s, a] - fd) < 1e-7 * max(1.0, abs(fd))
+
+
+def test_force_gauge_covariant(oracle):
+    rng = np.random.default_rng(6)
+    U = hot(D4, 32)
+    g = lt.random_su3(256, rng)
+    F = lt.coeffs_to_matrix(oracle.force(D4, U, COEFF))
+    Fg = lt.coeffs_to_matrix(oracle.force(D4, lt.gauge_transform(U, D4, g), COEFF))
+    assert np.abs(Fg - g[None] @ F @ lt.dg(g)[None]).max() < 1e-13
+
+
+def test_cold_force_zero(oracle):
+    F = oracle.force(D4, gen.cold_links(D4), COEFF)
+    assert np.abs(F).max() == 0.0
+
+
+def test_exp_update_and_kick(oracle):
+    U = hot(D4, 41)
+    P = gen.gaussian_momenta(D4, 42)
+    U2 = U.copy()
+    oracle.exp_update(D4, U2, P, 0.02)
+    MU, M2 = gen.links_to_matrices(U), gen.links_to_matrices(U2)
+    for mu, s in [(0, 0), (3, 255), (2, 77)]:
+        ref = expm(0.02 * lt.coeffs_to_matrix(P[mu, s])) @ MU[mu, s]
+        assert np.abs(M2[mu, s] - ref).max() < 1e-15
+    F = gen.gaussian_momenta(D4, 43)
+    P2 = P.copy()
+    oracle.kick(D4, P2, F, 0.3)
+    assert np.abs(P2 - (P + 0.3 * F)).max() == 0.0
+    assert abs(oracle.momenta_dot(D4, P) - np.sum(P * P)) < 1e-10
+
+
+# ---------------------------------------------------------------- Wilson flow
+
+
+def test_flow_force_is_scaled_md_force(oracle):
+    # add_force!(plaqonly) (AbstractGaugefields.jl:2717-2762): TA(U V^dag) with V the
+    # loops_staple_prime sum equals -(NC/coeff) times the FD-pinned MD force
+    U = hot(D4, 51)
+    assert np.abs(oracle.flow_force(D4, U) + 3.0 / COEFF * oracle.force(D4, U, COEFF)).max() < 1e-13
+
+
+def test_flow_cold_stays_cold_and_energy_decreases(oracle):
+    U = gen.cold_links(D4)
+    E = oracle.flow(D4, U, 0.01, 5, 1)
+    assert np.all(E == 0.0)
+    U = alg_links(oracle, D4, 0.4, 52)
+    E0 = oracle.energy_density(D4, U)
+    E = oracle.flow(D4, U, 0.01, 40, 5)
+    assert E0 > E[0] and np.all(np.diff(E) < 0)
+
+
+def test_flow_rk3_third_order(oracle):
+    # gradientflow.jl:171-238 is Luscher's 3rd-order RK: halving eps cuts the global
+    # error ~8x; compare against a 4x finer run
+    U0 = alg_links(oracle, D4, 0.4, 53)
+    runs = {}
+    for n in (10, 20, 80):
+        U = U0.copy()
+        oracle.flow(D4, U, 0.1 / n, n, 0)
+        runs[n] = U
+    e10 = np.abs(runs[10] - runs[80]).max()
+    e20 = np.abs(runs[20] - runs[80]).max()
+    assert 5.0 < e10 / e20 < 10.5
+
+
+def test_flow_gauge_covariant(oracle):
+    rng = np.random.default_rng(7)
+    U = alg_links(oracle, D4, 0.4, 54)
+    g = lt.random_su3(256, rng)
+    Ug = lt.gauge_transform(U, D4, g)
+    oracle.flow(D4, U, 0.02, 3, 0)
+    oracle.flow(D4, Ug, 0.02, 3, 0)
+    assert np.abs(Ug - lt.gauge_transform(U, D4, g)).max() < 1e-13
+
+
+# ---------------------------------------------------------------- stout
+
+
+def test_stout_rho_zero_identity(oracle):
+    U = hot(D4, 61)
+    assert np.abs(oracle.stout_forward(D4, U, 0.0) - U).max() == 0.0
+
+
+def test_stout_forward_vs_numpy(oracle):
+    dims = (2, 4, 2, 4)
+    U = hot(dims, 62)
+    M = lt.field(U, dims)
+    out = lt.field(oracle.stout_forward(dims, U, 0.1), dims)
+    for mu in range(4):
+        C = 0
+        for nu in range(4):
+            if nu == mu:
+                continue
+            C = C + M[nu] @ lt.sh(M[mu], nu) @ lt.dg(lt.sh(M[nu], mu))
+            Mnm = lt.sh(M[nu], nu, -1)
+            C = C + lt.dg(Mnm) @ lt.sh(M[mu], nu, -1) @ lt.sh(Mnm, mu)
+        Q = lt.ta_matrix(0.1 * C @ lt.dg(M[mu]))
+        flat = Q.reshape(-1, 3, 3)
+        E = np.array([expm(q) for q in flat]).reshape(Q.shape)
+        assert np.abs(out[mu] - E @ M[mu]).max() < 1e-14
+
+
+def test_stout_forward_gauge_covariant(oracle):
+    rng = np.random.default_rng(8)
+    U = hot(D4, 63)
+    g = lt.random_su3(256, rng)
+    a = oracle.stout_smear(D4, lt.gauge_transform(U, D4, g), 3, 0.12)
+    b = lt.gauge_transform(oracle.stout_smear(D4, U, 3, 0.12), D4, g)
+    assert np.abs(a - b).max() < 1e-13
+
+
+@pytest.mark.parametrize("nlev,rho,sigma", [(1, 0.1, 0.3), (3, 0.1, 0.3), (2, 0.12, 1.0)])
+def test_stout_force_vs_finite_difference(oracle, nlev, rho, sigma):
+    # stout HMC force (test/HMCstout_test_nowing.jl:99-118) = -dV_smeared/domega
+    U = alg_links(oracle, D4, sigma, 70 + nlev)
+    F = oracle.stout_force(D4, U, nlev, rho, COEFF)
+    rng = np.random.default_rng(9 + nlev)
+    pot = lambda W: oracle.stout_potential(D4, W, nlev, rho, COEFF)  # noqa: E731
+    for _ in range(64 if nlev == 3 else 24):
+        mu, s, a = rng.integers(4), rng.integers(256), rng.integers(8)
+        fd = _fd_force(pot, U, mu, s, a)
+        assert abs(F[mu, s, a] - fd) < 2e-7 * max(1.0, abs(fd))
+
+
+def test_stout_zero_levels_is_plain_force(oracle):
+    U = hot(D4, 81)
+    assert np.abs(oracle.stout_force(D4, U, 0, 0.1, COEFF) - oracle.force(D4, U, COEFF)).max() < 1e-15
+
+
+# ---------------------------------------------------------------- MD
+
+
+def test_md_cold_start(oracle):
+    # test/md_driver.jl:258-282: cold links, zero momenta -> dH == 0, plaquette stays 1
+    U = gen.cold_links(D4)
+    P = np.zeros((4, 256, 8))
+    H = oracle.md_trajectory(D4, U, P, COEFF, 5, 1.0, "QPQ")
+    assert H[2] == 0.0
+    assert oracle.plaquette_sum(D4, U) / (18 * 256) == 1.0
+
+
+@pytest.mark.parametrize("integ", ["QPQ", "PQP", "2MN"])
+def test_md_reversibility(oracle, integ):
+    # test/md_driver.jl:328-368: forward then trajectory_length -> -tau returns, < 2e-12
+    U0 = hot(D4, 91)
+    P0 = gen.gaussian_momenta(D4, 92)
+    U, P = U0.copy(), P0.copy()
+    oracle.md_trajectory(D4, U, P, COEFF, 5, 0.2, integ)
+    oracle.md_trajectory(D4, U, P, COEFF, 5, -0.2, integ)
+    assert np.abs(U - U0).max() < 2e-12 and np.abs(P - P0).max() < 2e-12
+
+
+def test_md_dH_scales_as_eps_squared(oracle):
+    U0 = alg_links(oracle, D4, 0.3, 93)
+    P0 = gen.gaussian_momenta(D4, 94)
+    dH = []
+    for steps in (10, 20):
+        U, P = U0.copy(), P0.copy()
+        dH.append(oracle.md_trajectory(D4, U, P, COEFF, steps, 0.5, "QPQ")[2])
+    assert 3.0 < dH[0] / dH[1] < 5.0
+
+
+def test_md_stout_reversibility(oracle):
+    U0 = alg_links(oracle, D4, 0.3, 95)
+    P0 = gen.gaussian_momenta(D4, 96)
+    U, P = U0.copy(), P0.copy()
+    H = oracle.md_trajectory(D4, U, P, COEFF, 3, 0.1, "2MN", nlev=2, rho=0.1)
+    oracle.md_trajectory(D4, U, P, COEFF, 3, -0.1, "2MN", nlev=2, rho=0.1)
+    assert np.abs(U - U0).max() < 2e-12
+    assert abs(H[2]) < 0.5
