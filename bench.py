@@ -1,0 +1,204 @@
+"""Benchmark of the SU(3) gauge-link hot path on one MI355X.
+
+Workload (the kernel-level plain-path configuration at its largest size): SU(3) 32^4
+periodic, FP64, links U = exp(i sum_a omega_a T_a) with seeded omega_a ~ N(0, 0.3^2),
+Gaussian momenta, Wilson action beta = 6.0.  One step = one fused MD update of every link
+(staple force + momentum kick + Cayley-Hamilton exp link update, one kernel).
+value = link-updates/s = 4 V / (time per step).
+
+Also reported: the per-kernel roofline of that kernel (algorithmic bytes / measured launch
+time, HIP events on the library's stream), the plaquette and force kernels, the stout-HMC
+seconds per trajectory in FP64 and FP32-bulk form at 32^4 (3 stout levels, rho = 0.12,
+Omelyan 2MN, 16 steps, tau = 1), and the CPU oracle timed on a bounded sample of the same
+workload on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import __graft_entry__  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+DIMS = (32, 32, 32, 32)
+BETA = 6.0
+COEFF = BETA / 2
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_lattice(gf, dims, precision, seed_u=20260, seed_p=20261, sigma=0.3):
+    gen = gf.generators
+    L = gf.Lattice(dims, precision=precision)
+    L.links_from_algebra(gen.algebra_coefficients(dims, sigma, seed_u))
+    L.upload_momenta(gen.gaussian_momenta(dims, seed_p))
+    return L
+
+
+def time_kernel(L, fn, reps):
+    """average duration (ms) of fn() over reps launches, HIP events on the library stream"""
+    L.sync()
+    L.event_record(2)
+    for _ in range(reps):
+        fn()
+    L.event_record(3)
+    return L.event_elapsed_ms(2, 3) / reps
+
+
+def cpu_baseline(dims, updates):
+    import oracle.gforacle as orc
+
+    gf = __graft_entry__.load_package()
+    gen = gf.generators
+    U = gen.cold_links(dims)
+    orc.exp_update(dims, U, gen.algebra_coefficients(dims, 0.3, 20260), 1.0)
+    P = gen.gaussian_momenta(dims, 20261)
+    V = int(np.prod(dims))
+    F = np.zeros((4, V, 8))
+    t0 = time.perf_counter()
+    for _ in range(updates):
+        F = orc.force(dims, U, COEFF)
+        orc.kick(dims, P, F, 0.02)
+        orc.exp_update(dims, U, P, 0.02)
+    dt = time.perf_counter() - t0
+    return {
+        "value": 4 * V * updates / dt,
+        "unit": "link-updates/s",
+        "cores": orc.get_threads(),
+        "kind": "port",
+        "sample": f"C oracle (OpenMP, FP64), {updates} MD updates (force + kick + exp) of a "
+        f"{'x'.join(map(str, dims))} lattice, {dt:.1f} s",
+    }
+
+
+def stout_hmc(gf, precision, ntraj, dims=DIMS):
+    """s/traj of 3-level stout HMC (rho 0.12, 2MN, 16 steps, tau 1) + per-trajectory dH"""
+    L = make_lattice(gf, dims, precision, seed_u=777, seed_p=778)
+    L.stout_config(3, 0.12, "store")
+    gen = gf.generators
+    dH, times = [], []
+    for k in range(ntraj + 1):
+        L.upload_momenta(gen.gaussian_momenta(dims, 1000 + k))
+        L.sync()
+        t0 = time.perf_counter()
+        H = L.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True)
+        L.sync()
+        if k > 0:  # first trajectory is warm-up
+            times.append(time.perf_counter() - t0)
+            dH.append(H[2])
+    plaq = L.plaquette()
+    L.close()
+    return {"s_per_traj": float(np.mean(times)), "dH": dH, "plaquette": plaq}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-stout", action="store_true", help="skip the stout-HMC s/traj lines")
+    ap.add_argument("--stout-traj", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the lattice would shard by domain decomposition; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    gf = __graft_entry__.load_package()
+    V = int(np.prod(DIMS))
+    log(f"[bench] 32^4 FP64 workload, V={V}")
+    L = make_lattice(gf, DIMS, "f64")
+    step = lambda: L.md_update(COEFF, 0.02, 0.02)  # noqa: E731
+    for _ in range(args.warmup):
+        step()
+    L.sync()
+    log(f"[bench] warmup {args.warmup} done")
+    t0 = time.perf_counter()
+    L.event_record(0)
+    for _ in range(args.steps):
+        step()
+    L.event_record(1)
+    L.sync()
+    wall = time.perf_counter() - t0
+    ms_ev = L.event_elapsed_ms(0, 1)
+    ms_per_step = ms_ev / args.steps
+    log(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step (wall {wall*1e3/args.steps:.4f})")
+
+    # algorithmic bytes per site: U in 72, P in 32, P out 32, U out 72 reals
+    bytes_md = 208 * 8 * V
+    achieved = bytes_md / (ms_per_step * 1e-3) / 1e9
+    # companion kernels
+    ms_force = time_kernel(L, lambda: L.force_kick(COEFF, 0.0), 10)
+    ms_exp = time_kernel(L, lambda: L.exp_update(0.02), 10)
+    ms_plaq = time_kernel(L, lambda: L.plaquette_sum(), 10)
+    plaq = L.plaquette()
+    L.close()
+    kernels = {
+        "md_update_fused": {"ms": ms_per_step, "bytes_per_site": 1664,
+                            "GBps": achieved},
+        "force_kick": {"ms": ms_force, "bytes_per_site": 1088,
+                       "GBps": 1088 * V / (ms_force * 1e-3) / 1e9},
+        "exp_update": {"ms": ms_exp, "bytes_per_site": 1408,
+                       "GBps": 1408 * V / (ms_exp * 1e-3) / 1e9},
+        "plaquette": {"ms_incl_readback": ms_plaq, "bytes_per_site": 576,
+                      "GBps": 576 * V / (ms_plaq * 1e-3) / 1e9},
+    }
+    log(f"[bench] kernels {json.dumps(kernels)}")
+
+    stout = None
+    if not args.no_stout:
+        s64 = stout_hmc(gf, "f64", args.stout_traj)
+        log(f"[bench] stout fp64 {s64}")
+        s32 = stout_hmc(gf, "f32", args.stout_traj)
+        log(f"[bench] stout fp32 {s32}")
+        stout = {
+            "lattice": "32^4", "levels": 3, "rho": 0.12, "integrator": "2MN", "steps": 16, "tau": 1.0,
+            "fp64_s_per_traj": s64["s_per_traj"], "fp32_s_per_traj": s32["s_per_traj"],
+            "speedup_fp32_over_fp64": s64["s_per_traj"] / s32["s_per_traj"],
+            "dH_fp64": s64["dH"], "dH_fp32": s32["dH"],
+            "max_abs_dH_diff": float(np.max(np.abs(np.array(s64["dH"]) - np.array(s32["dH"])))),
+            "plaquette_diff": abs(s64["plaquette"] - s32["plaquette"]),
+        }
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(DIMS, 2)
+        log(f"[bench] cpu {cpu}")
+
+    out = {
+        "metric": "SU(3) link-updates/s (fused force + kick + exp MD update)",
+        "value": 4 * V / (ms_per_step * 1e-3),
+        "unit": "link-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded omega_a ~ N(0, 0.3^2) links, Gaussian momenta)",
+        "config": {"workload": "SU(3) 32^4 periodic, Wilson beta=6.0, FP64, fused MD update",
+                   "lattice": "32x32x32x32", "sites": V},
+        "site_updates_per_s": V / (ms_per_step * 1e-3),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "kernels": kernels,
+        "stout_hmc": stout,
+        "plaquette": plaq,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

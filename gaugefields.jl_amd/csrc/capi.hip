@@ -1,0 +1,647 @@
+// capi.hip -- the C ABI of libgfhip.so: context, device buffers, host-side MD/flow/stout drivers.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <utility>
+
+#include "../../include/gfhip.h"
+#include "internal.hpp"
+
+using namespace gf;
+
+#define GF_VERSION 1
+#define GF_MAX_LEV 8
+#define GF_NEVENTS 16
+
+struct gf_ctx {
+    int dims[4] = {0, 0, 0, 0};
+    Geo geo{};
+    int V = 0;
+    int prec = GF_F64;
+    int device = 0;
+    size_t esz = 8;  // bytes per real
+    hipStream_t stream = nullptr;
+    // link-shaped fields (72 reals/site) and momentum-shaped fields (32 reals/site)
+    void* U = nullptr;     // current links
+    void* Ualt = nullptr;  // ping-pong partner of U
+    void* Ubak = nullptr;  // Metropolis backup
+    void* P = nullptr;     // momenta
+    void* F = nullptr;     // force
+    void* A = nullptr;     // flow accumulator
+    // stout
+    int nlev = 0;
+    double rho = 0.0;
+    int smode = GF_STOUT_STORE;
+    void* lev[GF_MAX_LEV] = {};  // smeared levels 1..nlev
+    void* Cst[GF_MAX_LEV] = {};  // stored C per level (input level index)
+    void* D = nullptr;
+    void* D2 = nullptr;
+    void* G = nullptr;
+    // scratch for layout conversion and reductions
+    double* staging = nullptr;  // 72 V doubles
+    double* partial = nullptr;
+    double* dres = nullptr;   // device scalar
+    double* hres = nullptr;   // pinned host scalar
+    size_t bytes = 0;
+    hipEvent_t ev[GF_NEVENTS] = {};
+    std::string err;
+};
+
+namespace {
+
+int fail(gf_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                                  \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess) return fail((c), GF_EDEVICE, "%s: %s", #expr, hipGetErrorString(_e));      \
+    } while (0)
+
+#define CTXCHK(c)                                                                                        \
+    do {                                                                                                 \
+        if (!(c)) return GF_EINVAL;                                                                      \
+        hipError_t _e = hipSetDevice((c)->device);                                                       \
+        if (_e != hipSuccess) return fail((c), GF_EDEVICE, "hipSetDevice: %s", hipGetErrorString(_e));   \
+    } while (0)
+
+size_t link_bytes(const gf_ctx* c) { return (size_t)72 * c->V * c->esz; }
+size_t mom_bytes(const gf_ctx* c) { return (size_t)32 * c->V * c->esz; }
+
+int alloc(gf_ctx* c, void** p, size_t n) {
+    if (*p) return GF_OK;
+    hipError_t e = hipMalloc(p, n);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, GF_ENOMEM, "hipMalloc(%zu bytes): %s", n, hipGetErrorString(e));
+    }
+    c->bytes += n;
+    return GF_OK;
+}
+void release(gf_ctx* c, void** p, size_t n) {
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+        c->bytes -= n;
+    }
+}
+
+#define TRY(x)                          \
+    do {                                \
+        int _r = (x);                   \
+        if (_r != GF_OK) return _r;     \
+    } while (0)
+
+int need_staging(gf_ctx* c) { return alloc(c, (void**)&c->staging, (size_t)72 * c->V * sizeof(double)); }
+
+// typed dispatch: f.template operator()<T>()
+template <typename F>
+int dispatch(gf_ctx* c, F&& f) {
+    return c->prec == GF_F64 ? f((double*)nullptr) : f((float*)nullptr);
+}
+
+int read_scalar(gf_ctx* c, double* out) {
+    HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = *c->hres;
+    return GF_OK;
+}
+
+int plaq_of(gf_ctx* c, void* U, double* sum) {
+    TRY(dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_plaquette<T>(c->geo, (const T*)U, c->partial, c->dres, c->stream));
+        return GF_OK;
+    }));
+    return read_scalar(c, sum);
+}
+
+void swap_links(gf_ctx* c) { std::swap(c->U, c->Ualt); }
+
+// ---- stout helpers -------------------------------------------------------------------
+int stout_alloc(gf_ctx* c) {
+    for (int l = 0; l < c->nlev; l++) {
+        TRY(alloc(c, &c->lev[l], link_bytes(c)));
+        if (c->smode == GF_STOUT_STORE) TRY(alloc(c, &c->Cst[l], link_bytes(c)));
+    }
+    TRY(alloc(c, &c->D, link_bytes(c)));
+    TRY(alloc(c, &c->D2, link_bytes(c)));
+    TRY(alloc(c, &c->G, link_bytes(c)));
+    return GF_OK;
+}
+const void* level_in(gf_ctx* c, int l) { return l == 0 ? c->U : c->lev[l - 1]; }
+
+int stout_forward_all(gf_ctx* c) {
+    return dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        for (int l = 0; l < c->nlev; l++)
+            HIPCHK(c, launch_stout_forward<T>(c->geo, (const T*)level_in(c, l), (T*)c->lev[l],
+                                              c->smode == GF_STOUT_STORE ? (T*)c->Cst[l] : nullptr, c->rho,
+                                              c->stream));
+        return GF_OK;
+    });
+}
+
+// D (dS/dU of the top level) -> D holds dS/dU_bare afterwards
+int stout_backward_all(gf_ctx* c) {
+    return dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        for (int l = c->nlev - 1; l >= 0; l--) {
+            HIPCHK(c, launch_stout_backward<T>(c->geo, (const T*)level_in(c, l),
+                                               c->smode == GF_STOUT_STORE ? (const T*)c->Cst[l] : nullptr,
+                                               (const T*)c->D, (T*)c->D2, (T*)c->G, c->rho, c->stream));
+            std::swap(c->D, c->D2);
+        }
+        return GF_OK;
+    });
+}
+
+const void* top_links(gf_ctx* c) { return c->nlev > 0 ? c->lev[c->nlev - 1] : c->U; }
+
+// mode FORCE_STORE -> F, FORCE_KICK -> P += eps F
+int stout_force_impl(gf_ctx* c, double coeff, int mode, double eps) {
+    TRY(stout_forward_all(c));
+    return dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_dsdu<T>(c->geo, (const T*)top_links(c), coeff, (T*)c->D, c->stream));
+        int r = stout_backward_all(c);
+        if (r != GF_OK) return r;
+        HIPCHK(c, launch_force_from_deriv<T>(mode, c->geo, (const T*)c->U, (const T*)c->D,
+                                              mode == FORCE_STORE ? (T*)c->F : (T*)c->P, eps, c->stream));
+        return GF_OK;
+    });
+}
+
+int kick(gf_ctx* c, double coeff, double eps) {
+    if (c->nlev > 0) return stout_force_impl(c, coeff, FORCE_KICK, eps);
+    return dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_force<T>(FORCE_KICK, c->geo, (const T*)c->U, (T*)c->P, nullptr, coeff, eps, 0.0, c->stream));
+        return GF_OK;
+    });
+}
+
+int expupd(gf_ctx* c, double eps) {
+    return dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_exp_update<T>(c->geo, (const T*)c->U, (T*)c->U, (const T*)c->P, eps, c->stream));
+        return GF_OK;
+    });
+}
+
+// kick then exp update; one fused kernel for the plain action
+int kick_exp(gf_ctx* c, double coeff, double eps_p, double eps_u) {
+    if (c->nlev > 0) {
+        TRY(stout_force_impl(c, coeff, FORCE_KICK, eps_p));
+        return expupd(c, eps_u);
+    }
+    TRY(dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_force<T>(FORCE_KICK_EXP, c->geo, (const T*)c->U, (T*)c->P, (T*)c->Ualt, coeff, eps_p, eps_u,
+                                  c->stream));
+        return GF_OK;
+    }));
+    swap_links(c);
+    return GF_OK;
+}
+
+int hamiltonian(gf_ctx* c, double coeff, double* H) {
+    double pp = 0.0, plaq = 0.0;
+    TRY(dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_momenta_dot<T>(c->geo, (const T*)c->P, c->partial, c->dres, c->stream));
+        return GF_OK;
+    }));
+    TRY(read_scalar(c, &pp));
+    if (c->nlev > 0) TRY(stout_forward_all(c));
+    TRY(plaq_of(c, (void*)top_links(c), &plaq));
+    *H = -2.0 * coeff * plaq / 3.0 + 0.5 * pp;
+    return GF_OK;
+}
+
+}  // namespace
+
+// =====================================================================================
+extern "C" {
+
+int gf_version(void) { return GF_VERSION; }
+
+const char* gf_last_error(const gf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
+    if (!dims || !out) return GF_EINVAL;
+    *out = nullptr;
+    if (precision != GF_F64 && precision != GF_F32) return GF_EINVAL;
+    long long V = 1;
+    for (int m = 0; m < 4; m++) {
+        if (dims[m] <= 0) return GF_EINVAL;
+        V *= dims[m];
+    }
+    // 72 reals per site must stay addressable with 32-bit site indices
+    if (V * 72 > (1LL << 31) - 1) return GF_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return GF_EDEVICE;
+    gf_ctx* c = new gf_ctx();
+    for (int m = 0; m < 4; m++) c->dims[m] = dims[m];
+    c->V = (int)V;
+    c->geo.V = (int)V;
+    int st = 1;
+    for (int m = 0; m < 4; m++) {
+        c->geo.n[m] = dims[m];
+        c->geo.st[m] = st;
+        st *= dims[m];
+    }
+    c->prec = precision;
+    c->esz = precision == GF_F64 ? 8 : 4;
+    c->device = device;
+    auto bail = [&](int code) {
+        gf_destroy(c);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(GF_EDEVICE);
+    for (int i = 0; i < GF_NEVENTS; i++)
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) return bail(GF_EDEVICE);
+    if (alloc(c, &c->U, link_bytes(c)) || alloc(c, &c->Ualt, link_bytes(c)) || alloc(c, &c->P, mom_bytes(c)) ||
+        alloc(c, &c->F, mom_bytes(c)) || alloc(c, (void**)&c->partial, sizeof(double) * grid_for(c->V)) ||
+        alloc(c, (void**)&c->dres, sizeof(double)))
+        return bail(GF_ENOMEM);
+    if (hipHostMalloc((void**)&c->hres, sizeof(double), hipHostMallocDefault) != hipSuccess) return bail(GF_ENOMEM);
+    int r = gf_set_cold(c);
+    if (r == GF_OK) r = gf_clear_momenta(c);
+    if (r == GF_OK) r = gf_sync(c);
+    if (r != GF_OK) return bail(r);
+    *out = c;
+    return GF_OK;
+}
+
+int gf_destroy(gf_ctx* c) {
+    if (!c) return GF_EINVAL;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void** bufs[] = {&c->U, &c->Ualt, &c->Ubak, &c->P, &c->F, &c->A, &c->D, &c->D2, &c->G};
+    for (void** b : bufs)
+        if (*b) (void)hipFree(*b);
+    for (int l = 0; l < GF_MAX_LEV; l++) {
+        if (c->lev[l]) (void)hipFree(c->lev[l]);
+        if (c->Cst[l]) (void)hipFree(c->Cst[l]);
+    }
+    if (c->staging) (void)hipFree(c->staging);
+    if (c->partial) (void)hipFree(c->partial);
+    if (c->dres) (void)hipFree(c->dres);
+    if (c->hres) (void)hipHostFree(c->hres);
+    for (int i = 0; i < GF_NEVENTS; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GF_OK;
+}
+
+int gf_sync(gf_ctx* c) {
+    CTXCHK(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GF_OK;
+}
+
+int gf_info(const gf_ctx* c, int* dims, int* precision, size_t* device_bytes) {
+    if (!c) return GF_EINVAL;
+    if (dims)
+        for (int m = 0; m < 4; m++) dims[m] = c->dims[m];
+    if (precision) *precision = c->prec;
+    if (device_bytes) *device_bytes = c->bytes;
+    return GF_OK;
+}
+
+int gf_set_cold(gf_ctx* c) {
+    CTXCHK(c);
+    return dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_set_identity<T>((T*)c->U, c->V, c->stream));
+        return GF_OK;
+    });
+}
+
+int gf_upload_links(gf_ctx* c, const double* ref) {
+    CTXCHK(c);
+    if (!ref) return fail(c, GF_EINVAL, "null links");
+    TRY(need_staging(c));
+    HIPCHK(c, hipMemcpyAsync(c->staging, ref, (size_t)72 * c->V * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TRY(dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_links_from_ref<T>(c->staging, (T*)c->U, c->V, c->stream));
+        return GF_OK;
+    }));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GF_OK;
+}
+
+static int download_linkfield(gf_ctx* c, const void* field, double* ref) {
+    TRY(need_staging(c));
+    TRY(dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_links_to_ref<T>((const T*)field, c->staging, c->V, c->stream));
+        return GF_OK;
+    }));
+    HIPCHK(c, hipMemcpyAsync(ref, c->staging, (size_t)72 * c->V * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GF_OK;
+}
+
+static int upload_linkfield(gf_ctx* c, void* field, const double* ref) {
+    TRY(need_staging(c));
+    HIPCHK(c, hipMemcpyAsync(c->staging, ref, (size_t)72 * c->V * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TRY(dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_links_from_ref<T>(c->staging, (T*)field, c->V, c->stream));
+        return GF_OK;
+    }));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GF_OK;
+}
+
+int gf_download_links(gf_ctx* c, double* ref) {
+    CTXCHK(c);
+    if (!ref) return fail(c, GF_EINVAL, "null links");
+    return download_linkfield(c, c->U, ref);
+}
+
+static int upload_mom(gf_ctx* c, void* field, const double* mom) {
+    TRY(need_staging(c));
+    HIPCHK(c, hipMemcpyAsync(c->staging, mom, (size_t)32 * c->V * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TRY(dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_momenta_from_ref<T>(c->staging, (T*)field, c->V, c->stream));
+        return GF_OK;
+    }));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GF_OK;
+}
+static int download_mom(gf_ctx* c, const void* field, double* mom) {
+    TRY(need_staging(c));
+    TRY(dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_momenta_to_ref<T>((const T*)field, c->staging, c->V, c->stream));
+        return GF_OK;
+    }));
+    HIPCHK(c, hipMemcpyAsync(mom, c->staging, (size_t)32 * c->V * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GF_OK;
+}
+
+int gf_upload_momenta(gf_ctx* c, const double* mom) {
+    CTXCHK(c);
+    if (!mom) return fail(c, GF_EINVAL, "null momenta");
+    return upload_mom(c, c->P, mom);
+}
+int gf_download_momenta(gf_ctx* c, double* mom) {
+    CTXCHK(c);
+    if (!mom) return fail(c, GF_EINVAL, "null momenta");
+    return download_mom(c, c->P, mom);
+}
+int gf_download_force(gf_ctx* c, double* force) {
+    CTXCHK(c);
+    if (!force) return fail(c, GF_EINVAL, "null force");
+    return download_mom(c, c->F, force);
+}
+int gf_clear_momenta(gf_ctx* c) {
+    CTXCHK(c);
+    return dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_fill<T>((T*)c->P, (size_t)32 * c->V, T(0), c->stream));
+        return GF_OK;
+    });
+}
+
+int gf_save_links(gf_ctx* c) {
+    CTXCHK(c);
+    TRY(alloc(c, &c->Ubak, link_bytes(c)));
+    HIPCHK(c, hipMemcpyAsync(c->Ubak, c->U, link_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+    return GF_OK;
+}
+int gf_restore_links(gf_ctx* c) {
+    CTXCHK(c);
+    if (!c->Ubak) return fail(c, GF_ESTATE, "gf_restore_links before gf_save_links");
+    HIPCHK(c, hipMemcpyAsync(c->U, c->Ubak, link_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+    return GF_OK;
+}
+
+int gf_plaquette(gf_ctx* c, double* sum) {
+    CTXCHK(c);
+    if (!sum) return GF_EINVAL;
+    return plaq_of(c, c->U, sum);
+}
+int gf_action(gf_ctx* c, double coeff, double* value) {
+    CTXCHK(c);
+    if (!value) return GF_EINVAL;
+    double s;
+    TRY(plaq_of(c, c->U, &s));
+    *value = 2.0 * coeff * s;
+    return GF_OK;
+}
+int gf_momenta_dot(gf_ctx* c, double* value) {
+    CTXCHK(c);
+    if (!value) return GF_EINVAL;
+    TRY(dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_momenta_dot<T>(c->geo, (const T*)c->P, c->partial, c->dres, c->stream));
+        return GF_OK;
+    }));
+    return read_scalar(c, value);
+}
+int gf_hamiltonian(gf_ctx* c, double coeff, double* H) {
+    CTXCHK(c);
+    if (!H) return GF_EINVAL;
+    return hamiltonian(c, coeff, H);
+}
+
+int gf_dsdu(gf_ctx* c, double coeff, double* ref) {
+    CTXCHK(c);
+    TRY(alloc(c, &c->D, link_bytes(c)));
+    TRY(dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_dsdu<T>(c->geo, (const T*)c->U, coeff, (T*)c->D, c->stream));
+        return GF_OK;
+    }));
+    if (ref) return download_linkfield(c, c->D, ref);
+    return GF_OK;
+}
+
+int gf_force(gf_ctx* c, double coeff) {
+    CTXCHK(c);
+    if (c->nlev > 0) return stout_force_impl(c, coeff, FORCE_STORE, 0.0);
+    return dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_force<T>(FORCE_STORE, c->geo, (const T*)c->U, (T*)c->F, nullptr, coeff, 0.0, 0.0, c->stream));
+        return GF_OK;
+    });
+}
+int gf_force_kick(gf_ctx* c, double coeff, double eps) {
+    CTXCHK(c);
+    if (!std::isfinite(eps)) return fail(c, GF_EINVAL, "the momentum step size must be finite; got %g", eps);
+    return kick(c, coeff, eps);
+}
+int gf_exp_update(gf_ctx* c, double eps) {
+    CTXCHK(c);
+    if (!std::isfinite(eps)) return fail(c, GF_EINVAL, "the gauge-field step size must be finite; got %g", eps);
+    return expupd(c, eps);
+}
+int gf_md_update(gf_ctx* c, double coeff, double eps_p, double eps_u) {
+    CTXCHK(c);
+    if (!std::isfinite(eps_p) || !std::isfinite(eps_u)) return fail(c, GF_EINVAL, "non-finite step size");
+    return kick_exp(c, coeff, eps_p, eps_u);
+}
+
+int gf_md_trajectory(gf_ctx* c, double coeff, int integrator, double lambda, int steps, double tau, double* H) {
+    CTXCHK(c);
+    const int kind = integrator & 0xff;
+    const bool fused = (integrator & GF_FUSED) != 0;
+    if (steps <= 0) return fail(c, GF_EINVAL, "steps must be positive; got %d", steps);
+    if (!std::isfinite(tau)) return fail(c, GF_EINVAL, "trajectory_length must be finite; got %g", tau);
+    if (tau == 0.0) return fail(c, GF_EINVAL, "trajectory_length must not be zero");
+    if (kind != GF_QPQ && kind != GF_PQP && kind != GF_OMELYAN2MN)
+        return fail(c, GF_EINVAL, "unknown integrator %d", integrator);
+    double h0 = 0.0;
+    if (H) TRY(hamiltonian(c, coeff, &h0));
+    const double e = tau / steps;
+    if (!fused) {
+        for (int k = 0; k < steps; k++) {
+            if (kind == GF_QPQ) {
+                TRY(expupd(c, e / 2));
+                TRY(kick(c, coeff, e));
+                TRY(expupd(c, e / 2));
+            } else if (kind == GF_PQP) {
+                TRY(kick(c, coeff, e / 2));
+                TRY(expupd(c, e));
+                TRY(kick(c, coeff, e / 2));
+            } else {
+                TRY(kick(c, coeff, lambda * e));
+                TRY(expupd(c, 0.5 * e));
+                TRY(kick(c, coeff, (1.0 - 2.0 * lambda) * e));
+                TRY(expupd(c, 0.5 * e));
+                TRY(kick(c, coeff, lambda * e));
+            }
+        }
+    } else if (kind == GF_QPQ) {  // Q(e/2) [P(e) Q(e)]^(n-1) P(e) Q(e/2)
+        TRY(expupd(c, e / 2));
+        for (int k = 0; k < steps - 1; k++) TRY(kick_exp(c, coeff, e, e));
+        TRY(kick_exp(c, coeff, e, e / 2));
+    } else if (kind == GF_PQP) {  // [P(e/2) Q(e)] [P(e) Q(e)]^(n-1) P(e/2)
+        TRY(kick_exp(c, coeff, e / 2, e));
+        for (int k = 0; k < steps - 1; k++) TRY(kick_exp(c, coeff, e, e));
+        TRY(kick(c, coeff, e / 2));
+    } else {  // 2MN: P(le) Q(e/2) P((1-2l)e) Q(e/2) [P(2le) Q(e/2) P((1-2l)e) Q(e/2)]^(n-1) P(le)
+        for (int k = 0; k < steps; k++) {
+            TRY(kick_exp(c, coeff, (k == 0 ? 1.0 : 2.0) * lambda * e, 0.5 * e));
+            TRY(kick_exp(c, coeff, (1.0 - 2.0 * lambda) * e, 0.5 * e));
+        }
+        TRY(kick(c, coeff, lambda * e));
+    }
+    if (H) {
+        double h1 = 0.0;
+        TRY(hamiltonian(c, coeff, &h1));
+        H[0] = h0;
+        H[1] = h1;
+        H[2] = h1 - h0;
+    }
+    return GF_OK;
+}
+
+int gf_energy_density(gf_ctx* c, double* E) {
+    CTXCHK(c);
+    if (!E) return GF_EINVAL;
+    double s;
+    TRY(plaq_of(c, c->U, &s));
+    *E = 2.0 * (18.0 - s / (double)c->V);
+    return GF_OK;
+}
+
+int gf_flow(gf_ctx* c, double eps, int nsteps, int every, double* E_out) {
+    CTXCHK(c);
+    if (nsteps <= 0) return fail(c, GF_EINVAL, "steps must be positive; got %d", nsteps);
+    if (!(eps > 0.0) || !std::isfinite(eps)) return fail(c, GF_EINVAL, "step_size must be positive; got %g", eps);
+    if (every > 0 && !E_out) return fail(c, GF_EINVAL, "E_out is required when every > 0");
+    TRY(alloc(c, &c->A, mom_bytes(c)));
+    int k = 0;
+    for (int step = 1; step <= nsteps; step++) {
+        TRY(dispatch(c, [&](auto* tag) -> int {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            HIPCHK(c, launch_flow_stage<T>(c->geo, (const T*)c->U, (T*)c->Ualt, (T*)c->A, -eps / 4.0, 0.0, c->stream));
+            HIPCHK(c, launch_flow_stage<T>(c->geo, (const T*)c->Ualt, (T*)c->U, (T*)c->A, -8.0 * eps / 9.0, -17.0 / 9.0,
+                                           c->stream));
+            HIPCHK(c, launch_flow_stage<T>(c->geo, (const T*)c->U, (T*)c->Ualt, (T*)c->A, -3.0 * eps / 4.0, -1.0,
+                                           c->stream));
+            return GF_OK;
+        }));
+        swap_links(c);
+        if (every > 0 && step % every == 0) TRY(gf_energy_density(c, &E_out[k++]));
+    }
+    return GF_OK;
+}
+
+int gf_stout_config(gf_ctx* c, int nlev, double rho, int mode) {
+    CTXCHK(c);
+    if (nlev < 0 || nlev > GF_MAX_LEV) return fail(c, GF_EINVAL, "nlev must be in 0..%d; got %d", GF_MAX_LEV, nlev);
+    if (!std::isfinite(rho)) return fail(c, GF_EINVAL, "rho must be finite");
+    if (mode != GF_STOUT_STORE && mode != GF_STOUT_RECOMPUTE) return fail(c, GF_EINVAL, "unknown stout mode %d", mode);
+    // release stored C of a previous store-mode configuration if no longer needed
+    for (int l = 0; l < GF_MAX_LEV; l++)
+        if (l >= nlev || mode != GF_STOUT_STORE) release(c, &c->Cst[l], link_bytes(c));
+    for (int l = nlev; l < GF_MAX_LEV; l++) release(c, &c->lev[l], link_bytes(c));
+    c->nlev = nlev;
+    c->rho = rho;
+    c->smode = mode;
+    if (nlev > 0) TRY(stout_alloc(c));
+    return GF_OK;
+}
+
+int gf_stout_smear(gf_ctx* c, double* ref) {
+    CTXCHK(c);
+    if (c->nlev == 0) return fail(c, GF_ESTATE, "no stout levels configured");
+    TRY(stout_forward_all(c));
+    if (ref) return download_linkfield(c, c->lev[c->nlev - 1], ref);
+    return GF_OK;
+}
+
+int gf_stout_backward(gf_ctx* c, const double* dout_ref, double* din_ref) {
+    CTXCHK(c);
+    if (c->nlev == 0) return fail(c, GF_ESTATE, "no stout levels configured");
+    if (!dout_ref || !din_ref) return fail(c, GF_EINVAL, "null derivative buffer");
+    TRY(upload_linkfield(c, c->D, dout_ref));
+    TRY(stout_backward_all(c));
+    return download_linkfield(c, c->D, din_ref);
+}
+
+int gf_stout_force(gf_ctx* c, double coeff) {
+    CTXCHK(c);
+    if (c->nlev == 0) return gf_force(c, coeff);
+    return stout_force_impl(c, coeff, FORCE_STORE, 0.0);
+}
+int gf_stout_force_kick(gf_ctx* c, double coeff, double eps) {
+    CTXCHK(c);
+    if (!std::isfinite(eps)) return fail(c, GF_EINVAL, "the momentum step size must be finite; got %g", eps);
+    return kick(c, coeff, eps);
+}
+
+int gf_event_record(gf_ctx* c, int slot) {
+    CTXCHK(c);
+    if (slot < 0 || slot >= GF_NEVENTS) return fail(c, GF_EINVAL, "event slot %d out of range", slot);
+    HIPCHK(c, hipEventRecord(c->ev[slot], c->stream));
+    return GF_OK;
+}
+int gf_event_elapsed(gf_ctx* c, int a, int b, float* ms) {
+    CTXCHK(c);
+    if (a < 0 || a >= GF_NEVENTS || b < 0 || b >= GF_NEVENTS || !ms) return GF_EINVAL;
+    HIPCHK(c, hipEventSynchronize(c->ev[b]));
+    HIPCHK(c, hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+    return GF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+// k_flow_stout.hip -- Luscher RK3 Wilson-flow stage, stout smearing forward and backward.
+#include "internal.hpp"
+
+namespace gf {
+
+// ---------------------------------------------------------------------------
+// K7: one RK3 stage (flow!, gradientflow.jl:171-238, in the low-storage form):
+//   F = TA(W_mu V_mu^dag)                  (add_force! plaqonly, AbstractGaugefields.jl:2717-2762)
+//   A = alpha F + beta A
+//   Wout_mu = exp(A_mu) W_mu               (exp_aF_U!, AbstractGaugefields.jl:2810-2841)
+// stages (alpha, beta) = (-eps/4, 0), (-8eps/9, -17/9), (-3eps/4, -1)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restrict__ W, T* __restrict__ Wout,
+                                                       T* __restrict__ A, T alpha, T beta) {
+    const int s = site_index();
+    if (s >= g.V) return;
+    int x[4];
+    decode(g, s, x);
+    const size_t V = g.V;
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) {
+        const M3<T> sig = staple_sigma(g, W, V, s, x, mu);
+        const M3<T> w = ldl(W, V, mu, s);
+        T c[8];
+        ta_coeffs(mul(w, sig), c);
+        if (beta != T(0)) {
+            T a[8];
+            ldp(A, V, mu, s, a);
+#pragma unroll
+            for (int k = 0; k < 8; k++) c[k] = fma(alpha, c[k], beta * a[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; k++) c[k] = alpha * c[k];
+        }
+        stp(A, V, mu, s, c);
+        const M3<T> E = expi_herm(herm_from_coeffs(c, T(1)));
+        stl(Wout, V, mu, s, mul(E, w));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K5: stout forward level (forward!, stout_fast.jl:250-274; calc_C! :603-624)
+//   C_mu = rho V_mu = rho Sigma_mu^dag,  Q_mu = TA(C_mu U_mu^dag),  Uout_mu = exp(Q_mu) U_mu
+// STORE keeps C_mu for the backward pass.
+// ---------------------------------------------------------------------------
+template <typename T, bool STORE>
+__global__ __launch_bounds__(kBlock) void k_stout_forward(Geo g, const T* __restrict__ Uin, T* __restrict__ Uout,
+                                                          T* __restrict__ Cst, T rho) {
+    const int s = site_index();
+    if (s >= g.V) return;
+    int x[4];
+    decode(g, s, x);
+    const size_t V = g.V;
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) {
+        const M3<T> C = scal(rho, dag(staple_sigma(g, Uin, V, s, x, mu)));
+        const M3<T> u = ldl(Uin, V, mu, s);
+        const M3<T> E = exp_antiherm(ta_mat(mul_nd(C, u)));
+        stl(Uout, V, mu, s, mul(E, u));
+        if (STORE) stl(Cst, V, mu, s, C);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K6a: stout backward, site-local part (backward_dSdUαUβρ_add!, stout_fast.jl:317-407):
+//   Din_mu = Dout_mu e^{Q_mu} + (Lam_mu C_mu)^dag,   Lam_mu = TA(CdexpQdQ(U_mu Dout_mu, Q_mu)),
+//   G_mu   = U_mu^dag Lam_mu  (= dS/dC_mu)
+// STORED reads C_mu from the forward pass, otherwise it is recomputed from the staples.
+// ---------------------------------------------------------------------------
+template <typename T, bool STORED>
+__global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __restrict__ Uin,
+                                                             const T* __restrict__ Cst, const T* __restrict__ Dout,
+                                                             T* __restrict__ Din, T* __restrict__ G, T rho) {
+    const int s = site_index();
+    if (s >= g.V) return;
+    int x[4];
+    decode(g, s, x);
+    const size_t V = g.V;
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) {
+        const M3<T> C = STORED ? ldl(Cst, V, mu, s) : scal(rho, dag(staple_sigma(g, Uin, V, s, x, mu)));
+        const M3<T> u = ldl(Uin, V, mu, s);
+        const M3<T> Q = ta_mat(mul_nd(C, u));
+        const M3<T> Do = ldl(Dout, V, mu, s);
+        M3<T> D = mul(Do, exp_antiherm(Q));
+        const M3<T> Lam = ta_mat(cdexpqdq(mul(u, Do), Q));
+        stl(G, V, mu, s, mul_dn(u, Lam));
+        acc_dag(D, mul(Lam, C));
+        stl(Din, V, mu, s, D);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K6b: stout backward, staple-derivative part (calc_dSdUν_fromdSCμ_add!, stout_fast.jl:712-785):
+// for output direction a at site y and each plane b != a,
+//   Din_a(y) += rho [ U_b(y+a) (G_a(y+b) U_b(y)^dag + U_a(y+b)^dag G_b(y))
+//                   + U_b(y-b+a)^dag (G_a(y-b) U_b(y-b) + U_a(y-b)^dag G_b(y-b)^dag)
+//                   + G_b(y+a)^dag U_a(y+b)^dag U_b(y)^dag
+//                   + G_b(y+a-b) U_a(y-b)^dag U_b(y-b) ]
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* __restrict__ U,
+                                                               const T* __restrict__ G, T* __restrict__ Din, T rho) {
+    const int y = site_index();
+    if (y >= g.V) return;
+    int x[4];
+    decode(g, y, x);
+    const size_t V = g.V;
+#pragma unroll 1
+    for (int a = 0; a < 4; a++) {
+        M3<T> acc = zero3<T>();
+        const int ypa = y + dlt(g, x, a, 1);
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            if (b == a) continue;
+            int y0 = y, y1 = ypa, ypb = y + dlt(g, x, b, 1);
+            const int dmb = dlt(g, x, b, -1);
+            int ymb = y + dmb, ypamb = ypa + dmb;
+            asm volatile("" : "+v"(y0), "+v"(y1), "+v"(ypb), "+v"(ymb), "+v"(ypamb));  // no cross-plane merging
+            {  // U_b(y+a) [G_a(y+b) U_b(y)^dag + U_a(y+b)^dag G_b(y)]
+                M3<T> t = mul_nd(ldl(G, V, a, ypb), ldl(U, V, b, y0));
+                acc_mul_dn(t, ldl(U, V, a, ypb), ldl(G, V, b, y0));
+                acc_mul(acc, ldl(U, V, b, y1), t);
+            }
+            {  // U_b(y-b+a)^dag [G_a(y-b) U_b(y-b) + (G_b(y-b) U_a(y-b))^dag]
+                M3<T> t = mul(ldl(G, V, a, ymb), ldl(U, V, b, ymb));
+                acc_dag(t, mul(ldl(G, V, b, ymb), ldl(U, V, a, ymb)));
+                acc_mul_dn(acc, ldl(U, V, b, ypamb), t);
+            }
+            {  // (U_b(y) U_a(y+b) G_b(y+a))^dag
+                acc_dag(acc, mul(mul(ldl(U, V, b, y0), ldl(U, V, a, ypb)), ldl(G, V, b, y1)));
+            }
+            {  // G_b(y+a-b) U_a(y-b)^dag U_b(y-b)
+                acc_mul(acc, ldl(G, V, b, ypamb), mul_dn(ldl(U, V, a, ymb), ldl(U, V, b, ymb)));
+            }
+        }
+        M3<T> d = ldl(Din, V, a, y);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            d.m[k].r = fma(rho, acc.m[k].r, d.m[k].r);
+            d.m[k].i = fma(rho, acc.m[k].i, d.m[k].i);
+        }
+        stl(Din, V, a, y, d);
+    }
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+hipError_t launch_flow_stage(const Geo& g, const T* W, T* Wout, T* A, double alpha, double beta, hipStream_t st) {
+    k_flow_stage<T><<<grid_for(g.V), kBlock, 0, st>>>(g, W, Wout, A, (T)alpha, (T)beta);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_stout_forward(const Geo& g, const T* Uin, T* Uout, T* Cst, double rho, hipStream_t st) {
+    if (Cst)
+        k_stout_forward<T, true><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, Uout, Cst, (T)rho);
+    else
+        k_stout_forward<T, false><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, Uout, nullptr, (T)rho);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* G, double rho,
+                                 hipStream_t st) {
+    if (Cst)
+        k_stout_back_local<T, true><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, Cst, Dout, Din, G, (T)rho);
+    else
+        k_stout_back_local<T, false><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, G, (T)rho);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_stout_back_stencil<T><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, G, Din, (T)rho);
+    return hipGetLastError();
+}
+
+#define GF_INST2(T)                                                                                       \
+    template hipError_t launch_flow_stage<T>(const Geo&, const T*, T*, T*, double, double, hipStream_t);  \
+    template hipError_t launch_stout_forward<T>(const Geo&, const T*, T*, T*, double, hipStream_t);       \
+    template hipError_t launch_stout_backward<T>(const Geo&, const T*, const T*, const T*, T*, T*, double, \
+                                                 hipStream_t);
+
+GF_INST2(double)
+GF_INST2(float)
+
+}  // namespace gf

@@ -1,0 +1,355 @@
+// k_gauge.hip -- plaquette/action sums, staple derivative, fused force + momentum kick
+// (+ exp link update), exp link update, layout conversion.
+//
+// One thread per lattice site; each thread handles all four link directions of its site.
+// Neighbour links are re-read through L1/L2 (the XCD-aware block order keeps a site's
+// halo in the same XCD's L2); HBM sees each input plane once per launch.
+#include "internal.hpp"
+
+namespace gf {
+
+// ---------------------------------------------------------------------------
+// K1: plaquette sum  sum_{x, mu<nu} Re tr U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+// (calculate_Plaquette, AbstractGaugefields.jl:2684-2699)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_plaquette(Geo g, const T* __restrict__ U, double* __restrict__ partial) {
+    const int s = site_index();
+    double v = 0.0;
+    if (s < g.V) {
+        int x[4];
+        decode(g, s, x);
+        const size_t V = g.V;
+#pragma unroll
+        for (int mu = 0; mu < 3; mu++) {
+            const int spm = s + dlt(g, x, mu, 1);
+            const M3<T> umu = ldl(U, V, mu, s);
+#pragma unroll
+            for (int nu = mu + 1; nu < 4; nu++) {
+                int s0 = s, s1 = spm, spn = s + dlt(g, x, nu, 1);
+                asm volatile("" : "+v"(s0), "+v"(s1), "+v"(spn));  // no cross-plane load merging
+                const M3<T> a = mul(umu, ldl(U, V, nu, s1));                // U_mu(x) U_nu(x+mu)
+                const M3<T> b = mul(ldl(U, V, nu, s0), ldl(U, V, mu, spn));  // U_nu(x) U_mu(x+nu)
+                v += (double)retr_nd(a, b);
+            }
+        }
+    }
+    block_sum_store<kBlock>(v, partial);
+}
+
+// sum of p_a^2 over all links (p*p, TA_gaugefields_4D_serial.jl:106-129)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_momenta_dot(Geo g, const T* __restrict__ P, double* __restrict__ partial) {
+    const int s = site_index();
+    double v = 0.0;
+    if (s < g.V) {
+        const size_t V = g.V;
+#pragma unroll
+        for (int k = 0; k < 32; k++) {
+            const double p = (double)P[k * V + s];
+            v = fma(p, p, v);
+        }
+    }
+    block_sum_store<kBlock>(v, partial);
+}
+
+// deterministic final sum of per-block partials (one block)
+__global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict__ partial, int n,
+                                                       double* __restrict__ out) {
+    __shared__ double red[1024];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) v += partial[i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// K2: dSdU_mu = coeff * Sigma_mu for all four directions
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U, T coeff, T* __restrict__ D) {
+    const int s = site_index();
+    if (s >= g.V) return;
+    int x[4];
+    decode(g, s, x);
+    const size_t V = g.V;
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) stl(D, V, mu, s, scal(coeff, staple_sigma(g, U, V, s, x, mu)));
+}
+
+// K3 (+K4): F_mu = -(coeff/3) TA(U_mu Sigma_mu)   (md_force!, molecular_dynamics.jl:251-267)
+//   FORCE_STORE    : FP = F
+//   FORCE_KICK     : FP += eps_p F                    (update_momenta!, :539-551)
+//   FORCE_KICK_EXP : FP += eps_p F; Unew = exp(eps_u FP) U   (then update_gaugefields!, :513-531)
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U, T* __restrict__ FP,
+                                                  T* __restrict__ Unew, T fac, T eps_u) {
+    const int s = site_index();
+    if (s >= g.V) return;
+    int x[4];
+    decode(g, s, x);
+    const size_t V = g.V;
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) {
+        const M3<T> sig = staple_sigma(g, U, V, s, x, mu);
+        const M3<T> umu = ldl(U, V, mu, s);
+        T c[8];
+        ta_coeffs(mul(umu, sig), c);
+        if (MODE == FORCE_STORE) {
+#pragma unroll
+            for (int a = 0; a < 8; a++) c[a] *= fac;
+            stp(FP, V, mu, s, c);
+        } else {
+            T p[8];
+            ldp(FP, V, mu, s, p);
+#pragma unroll
+            for (int a = 0; a < 8; a++) p[a] = fma(fac, c[a], p[a]);
+            stp(FP, V, mu, s, p);
+            if (MODE == FORCE_KICK_EXP) {
+                const M3<T> E = expi_herm(herm_from_coeffs(p, eps_u));
+                stl(Unew, V, mu, s, mul(E, umu));
+            }
+        }
+    }
+}
+
+// K4: U_mu <- exp(eps P_mu) U_mu  (exptU! + mul!, molecular_dynamics.jl:513-531)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_exp_update(Geo g, const T* __restrict__ Uin, T* __restrict__ Uout,
+                                                       const T* __restrict__ P, T eps) {
+    const int s = site_index();
+    if (s >= g.V) return;
+    const size_t V = g.V;
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) {
+        T p[8];
+        ldp(P, V, mu, s, p);
+        const M3<T> E = expi_herm(herm_from_coeffs(p, eps));
+        stl(Uout, V, mu, s, mul(E, ldl(Uin, V, mu, s)));
+    }
+}
+
+// F_mu = -(1/3) TA(U_mu D_mu) from a matrix derivative D (stout force,
+// test/HMCstout_test_nowing.jl:99-118); store or kick
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_force_from_deriv(Geo g, const T* __restrict__ U, const T* __restrict__ D,
+                                                             T* __restrict__ FP, T eps) {
+    const int s = site_index();
+    if (s >= g.V) return;
+    const size_t V = g.V;
+    const T fac = T(-1.0 / 3.0);
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) {
+        T c[8];
+        ta_coeffs(mul(ldl(U, V, mu, s), ldl(D, V, mu, s)), c);
+        if (MODE == FORCE_STORE) {
+#pragma unroll
+            for (int a = 0; a < 8; a++) c[a] *= fac;
+            stp(FP, V, mu, s, c);
+        } else {
+            T p[8];
+            ldp(FP, V, mu, s, p);
+            const T f = eps * fac;
+#pragma unroll
+            for (int a = 0; a < 8; a++) p[a] = fma(f, c[a], p[a]);
+            stp(FP, V, mu, s, p);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// layout conversion between the reference's (3,3,X,Y,Z,T) complex column-major arrays
+// (FP64, per direction) and the SoA planes
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_links_from_ref(const double* __restrict__ ref, T* __restrict__ soa, int V) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= V) return;
+    const size_t Vs = V;
+    for (int mu = 0; mu < 4; mu++) {
+        const double* r = ref + ((size_t)mu * Vs + s) * 18;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int k = 3 * i + j, c = i + 3 * j;
+                soa[((size_t)mu * 18 + 2 * k) * Vs + s] = (T)r[2 * c];
+                soa[((size_t)mu * 18 + 2 * k + 1) * Vs + s] = (T)r[2 * c + 1];
+            }
+    }
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_links_to_ref(const T* __restrict__ soa, double* __restrict__ ref, int V) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= V) return;
+    const size_t Vs = V;
+    for (int mu = 0; mu < 4; mu++) {
+        double* r = ref + ((size_t)mu * Vs + s) * 18;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int k = 3 * i + j, c = i + 3 * j;
+                r[2 * c] = (double)soa[((size_t)mu * 18 + 2 * k) * Vs + s];
+                r[2 * c + 1] = (double)soa[((size_t)mu * 18 + 2 * k + 1) * Vs + s];
+            }
+    }
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_momenta_from_ref(const double* __restrict__ ref, T* __restrict__ soa, int V) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= V) return;
+    const size_t Vs = V;
+    for (int mu = 0; mu < 4; mu++)
+#pragma unroll
+        for (int a = 0; a < 8; a++) soa[((size_t)mu * 8 + a) * Vs + s] = (T)ref[((size_t)mu * Vs + s) * 8 + a];
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_momenta_to_ref(const T* __restrict__ soa, double* __restrict__ ref, int V) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= V) return;
+    const size_t Vs = V;
+    for (int mu = 0; mu < 4; mu++)
+#pragma unroll
+        for (int a = 0; a < 8; a++) ref[((size_t)mu * Vs + s) * 8 + a] = (double)soa[((size_t)mu * 8 + a) * Vs + s];
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_set_identity(T* __restrict__ U, int V) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= V) return;
+    const size_t Vs = V;
+    for (int mu = 0; mu < 4; mu++)
+#pragma unroll
+        for (int r = 0; r < 18; r++) U[((size_t)mu * 18 + r) * Vs + s] = (r == 0 || r == 8 || r == 16) ? T(1) : T(0);
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill(T* __restrict__ p, size_t n, T v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_axpby(T* __restrict__ y, const T* __restrict__ x, T a, T b, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename T>
+hipError_t launch_plaquette(const Geo& g, const T* U, double* partial, double* out, hipStream_t st) {
+    const int nb = grid_for(g.V);
+    k_plaquette<T><<<nb, kBlock, 0, st>>>(g, U, partial);
+    k_sum_partials<<<1, 1024, 0, st>>>(partial, nb, out);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_momenta_dot(const Geo& g, const T* P, double* partial, double* out, hipStream_t st) {
+    const int nb = grid_for(g.V);
+    k_momenta_dot<T><<<nb, kBlock, 0, st>>>(g, P, partial);
+    k_sum_partials<<<1, 1024, 0, st>>>(partial, nb, out);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_dsdu(const Geo& g, const T* U, double coeff, T* D, hipStream_t st) {
+    k_dsdu<T><<<grid_for(g.V), kBlock, 0, st>>>(g, U, (T)coeff, D);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_force(int mode, const Geo& g, const T* U, T* FP, T* Unew, double coeff, double eps_p,
+                        double eps_u, hipStream_t st) {
+    const int nb = grid_for(g.V);
+    const double fac = -coeff / 3.0;
+    switch (mode) {
+        case FORCE_STORE:
+            k_force<T, FORCE_STORE><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)fac, (T)eps_u);
+            break;
+        case FORCE_KICK:
+            k_force<T, FORCE_KICK><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)(fac * eps_p), (T)eps_u);
+            break;
+        case FORCE_KICK_EXP:
+            k_force<T, FORCE_KICK_EXP><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)(fac * eps_p), (T)eps_u);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_exp_update(const Geo& g, const T* Uin, T* Uout, const T* P, double eps, hipStream_t st) {
+    k_exp_update<T><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, Uout, P, (T)eps);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_force_from_deriv(int mode, const Geo& g, const T* U, const T* D, T* FP, double eps,
+                                   hipStream_t st) {
+    if (mode == FORCE_STORE)
+        k_force_from_deriv<T, FORCE_STORE><<<grid_for(g.V), kBlock, 0, st>>>(g, U, D, FP, (T)eps);
+    else if (mode == FORCE_KICK)
+        k_force_from_deriv<T, FORCE_KICK><<<grid_for(g.V), kBlock, 0, st>>>(g, U, D, FP, (T)eps);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_links_from_ref(const double* ref, T* soa, int V, hipStream_t st) {
+    k_links_from_ref<T><<<grid_for(V), kBlock, 0, st>>>(ref, soa, V);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_links_to_ref(const T* soa, double* ref, int V, hipStream_t st) {
+    k_links_to_ref<T><<<grid_for(V), kBlock, 0, st>>>(soa, ref, V);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_momenta_from_ref(const double* ref, T* soa, int V, hipStream_t st) {
+    k_momenta_from_ref<T><<<grid_for(V), kBlock, 0, st>>>(ref, soa, V);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_momenta_to_ref(const T* soa, double* ref, int V, hipStream_t st) {
+    k_momenta_to_ref<T><<<grid_for(V), kBlock, 0, st>>>(soa, ref, V);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_set_identity(T* U, int V, hipStream_t st) {
+    k_set_identity<T><<<grid_for(V), kBlock, 0, st>>>(U, V);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_fill(T* p, size_t n, T v, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_fill<T><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_axpby(T* y, const T* x, double a, double b, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    k_axpby<T><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(y, x, (T)a, (T)b, n);
+    return hipGetLastError();
+}
+
+#define GF_INST(T)                                                                                       \
+    template hipError_t launch_plaquette<T>(const Geo&, const T*, double*, double*, hipStream_t);        \
+    template hipError_t launch_momenta_dot<T>(const Geo&, const T*, double*, double*, hipStream_t);      \
+    template hipError_t launch_dsdu<T>(const Geo&, const T*, double, T*, hipStream_t);                   \
+    template hipError_t launch_force<T>(int, const Geo&, const T*, T*, T*, double, double, double,       \
+                                        hipStream_t);                                                    \
+    template hipError_t launch_exp_update<T>(const Geo&, const T*, T*, const T*, double, hipStream_t);   \
+    template hipError_t launch_force_from_deriv<T>(int, const Geo&, const T*, const T*, T*, double,      \
+                                                   hipStream_t);                                         \
+    template hipError_t launch_links_from_ref<T>(const double*, T*, int, hipStream_t);                   \
+    template hipError_t launch_links_to_ref<T>(const T*, double*, int, hipStream_t);                     \
+    template hipError_t launch_momenta_from_ref<T>(const double*, T*, int, hipStream_t);                 \
+    template hipError_t launch_momenta_to_ref<T>(const T*, double*, int, hipStream_t);                   \
+    template hipError_t launch_set_identity<T>(T*, int, hipStream_t);                                    \
+    template hipError_t launch_fill<T>(T*, size_t, T, hipStream_t);                                      \
+    template hipError_t launch_axpby<T>(T*, const T*, double, double, size_t, hipStream_t);
+
+GF_INST(double)
+GF_INST(float)
+
+}  // namespace gf

@@ -1,0 +1,488 @@
+// su3.hpp -- device-side SU(3) algebra for the gfx950 kernels.
+//
+// 3x3 complex matrices live in registers (row-major, 9 complex = 18 reals).  The storage
+// type T is double (FP64 form) or float (FP32-bulk form); the Cayley-Hamilton coefficients
+// f_j, b_ij and all global sums are always evaluated in double.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace gf {
+
+template <typename T>
+struct cx {
+    T r, i;
+};
+
+template <typename T>
+struct M3 {
+    cx<T> m[9];  // m[3*i + j] = element (i, j)
+};
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void cmac(cx<T>& c, const cx<T>& a, const cx<T>& b) {  // c += a b
+    c.r = fma(a.r, b.r, c.r);
+    c.r = fma(-a.i, b.i, c.r);
+    c.i = fma(a.r, b.i, c.i);
+    c.i = fma(a.i, b.r, c.i);
+}
+template <typename T>
+__device__ __forceinline__ void cmac_nc(cx<T>& c, const cx<T>& a, const cx<T>& b) {  // c += a conj(b)
+    c.r = fma(a.r, b.r, c.r);
+    c.r = fma(a.i, b.i, c.r);
+    c.i = fma(a.i, b.r, c.i);
+    c.i = fma(-a.r, b.i, c.i);
+}
+template <typename T>
+__device__ __forceinline__ void cmac_cn(cx<T>& c, const cx<T>& a, const cx<T>& b) {  // c += conj(a) b
+    c.r = fma(a.r, b.r, c.r);
+    c.r = fma(a.i, b.i, c.r);
+    c.i = fma(a.r, b.i, c.i);
+    c.i = fma(-a.i, b.r, c.i);
+}
+
+// ---------------------------------------------------------------------------
+// matrix products (all results by value; everything stays in registers)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ M3<T> zero3() {
+    M3<T> z;
+#pragma unroll
+    for (int k = 0; k < 9; k++) z.m[k].r = z.m[k].i = T(0);
+    return z;
+}
+template <typename T>
+__device__ __forceinline__ M3<T> ident3() {
+    M3<T> z = zero3<T>();
+    z.m[0].r = z.m[4].r = z.m[8].r = T(1);
+    return z;
+}
+template <typename T>
+__device__ __forceinline__ M3<T> mul(const M3<T>& a, const M3<T>& b) {  // a b
+    M3<T> c = zero3<T>();
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) cmac(c.m[3 * i + j], a.m[3 * i + k], b.m[3 * k + j]);
+    return c;
+}
+template <typename T>
+__device__ __forceinline__ M3<T> mul_nd(const M3<T>& a, const M3<T>& b) {  // a b^dag
+    M3<T> c = zero3<T>();
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) cmac_nc(c.m[3 * i + j], a.m[3 * i + k], b.m[3 * j + k]);
+    return c;
+}
+template <typename T>
+__device__ __forceinline__ M3<T> mul_dn(const M3<T>& a, const M3<T>& b) {  // a^dag b
+    M3<T> c = zero3<T>();
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) cmac_cn(c.m[3 * i + j], a.m[3 * k + i], b.m[3 * k + j]);
+    return c;
+}
+template <typename T>
+__device__ __forceinline__ void acc_mul(M3<T>& c, const M3<T>& a, const M3<T>& b) {  // c += a b
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) cmac(c.m[3 * i + j], a.m[3 * i + k], b.m[3 * k + j]);
+}
+template <typename T>
+__device__ __forceinline__ void acc_mul_nd(M3<T>& c, const M3<T>& a, const M3<T>& b) {  // c += a b^dag
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) cmac_nc(c.m[3 * i + j], a.m[3 * i + k], b.m[3 * j + k]);
+}
+template <typename T>
+__device__ __forceinline__ void acc_mul_dn(M3<T>& c, const M3<T>& a, const M3<T>& b) {  // c += a^dag b
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) cmac_cn(c.m[3 * i + j], a.m[3 * k + i], b.m[3 * k + j]);
+}
+template <typename T>
+__device__ __forceinline__ M3<T> dag(const M3<T>& a) {
+    M3<T> c;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            c.m[3 * i + j].r = a.m[3 * j + i].r;
+            c.m[3 * i + j].i = -a.m[3 * j + i].i;
+        }
+    return c;
+}
+template <typename T>
+__device__ __forceinline__ void acc(M3<T>& c, const M3<T>& a) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        c.m[k].r += a.m[k].r;
+        c.m[k].i += a.m[k].i;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void acc_dag(M3<T>& c, const M3<T>& a) {  // c += a^dag
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            c.m[3 * i + j].r += a.m[3 * j + i].r;
+            c.m[3 * i + j].i -= a.m[3 * j + i].i;
+        }
+}
+template <typename T>
+__device__ __forceinline__ M3<T> scal(T s, const M3<T>& a) {
+    M3<T> c;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        c.m[k].r = s * a.m[k].r;
+        c.m[k].i = s * a.m[k].i;
+    }
+    return c;
+}
+// Re tr(a b^dag) = sum_ij Re(a_ij conj(b_ij))
+template <typename T>
+__device__ __forceinline__ T retr_nd(const M3<T>& a, const M3<T>& b) {
+    T s = T(0);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        s = fma(a.m[k].r, b.m[k].r, s);
+        s = fma(a.m[k].i, b.m[k].i, s);
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// su(3) projections (basis i lambda_a / 2, order of TA_gaugefields_4D_serial.jl:243-260)
+// ---------------------------------------------------------------------------
+#define GF_SR3I 0.57735026918962576451
+
+// coefficients of TA(v) = (v - v^dag)/2 - tr(v - v^dag)/6
+template <typename T>
+__device__ __forceinline__ void ta_coeffs(const M3<T>& v, T c[8]) {
+    const T s3 = T(GF_SR3I);
+    c[0] = v.m[1].i + v.m[3].i;
+    c[1] = v.m[1].r - v.m[3].r;
+    c[2] = v.m[0].i - v.m[4].i;
+    c[3] = v.m[2].i + v.m[6].i;
+    c[4] = v.m[2].r - v.m[6].r;
+    c[5] = v.m[5].i + v.m[7].i;
+    c[6] = v.m[5].r - v.m[7].r;
+    c[7] = s3 * (v.m[0].i + v.m[4].i - T(2) * v.m[8].i);
+}
+
+// TA(v) as a matrix
+template <typename T>
+__device__ __forceinline__ M3<T> ta_mat(const M3<T>& v) {
+    M3<T> r;
+    T tri = (v.m[0].i + v.m[4].i + v.m[8].i) * T(1.0 / 3.0);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        r.m[4 * i].r = T(0);
+        r.m[4 * i].i = v.m[4 * i].i - tri;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = i + 1; j < 3; j++) {
+            T xr = T(0.5) * (v.m[3 * i + j].r - v.m[3 * j + i].r);
+            T xi = T(0.5) * (v.m[3 * i + j].i + v.m[3 * j + i].i);
+            r.m[3 * i + j].r = xr;
+            r.m[3 * i + j].i = xi;
+            r.m[3 * j + i].r = -xr;
+            r.m[3 * j + i].i = xi;
+        }
+    return r;
+}
+
+// Hermitian H = (t/2) sum_a c_a lambda_a
+template <typename T>
+__device__ __forceinline__ M3<T> herm_from_coeffs(const T c[8], T t) {
+    const T h = T(0.5) * t;
+    const T s3 = T(GF_SR3I);
+    T c1 = h * c[0], c2 = h * c[1], c3 = h * c[2], c4 = h * c[3];
+    T c5 = h * c[4], c6 = h * c[5], c7 = h * c[6], c8 = h * c[7];
+    M3<T> H;
+    H.m[0] = {c3 + s3 * c8, T(0)};
+    H.m[1] = {c1, -c2};
+    H.m[2] = {c4, -c5};
+    H.m[3] = {c1, c2};
+    H.m[4] = {-c3 + s3 * c8, T(0)};
+    H.m[5] = {c6, -c7};
+    H.m[6] = {c4, c5};
+    H.m[7] = {c6, c7};
+    H.m[8] = {T(-2) * s3 * c8, T(0)};
+    return H;
+}
+
+// ---------------------------------------------------------------------------
+// Cayley-Hamilton coefficients in double (Morningstar & Peardon, PRD 69 054501)
+// ---------------------------------------------------------------------------
+struct dc {
+    double r, i;
+};
+__device__ __forceinline__ dc dcm(dc a, dc b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ dc dcs(double s, dc a) { return {s * a.r, s * a.i}; }
+__device__ __forceinline__ dc dca(dc a, dc b) { return {a.r + b.r, a.i + b.i}; }
+__device__ __forceinline__ dc dcsub(dc a, dc b) { return {a.r - b.r, a.i - b.i}; }
+__device__ __forceinline__ dc dcconj(dc a) { return {a.r, -a.i}; }
+
+// c1 = tr(H^2)/2 and c0 = det H of a Hermitian traceless H, evaluated in double
+template <typename T>
+__device__ __forceinline__ void herm_invariants(const M3<T>& H, double& c0, double& c1) {
+    double h00 = H.m[0].r, h11 = H.m[4].r, h22 = H.m[8].r;
+    double ar = H.m[1].r, ai = H.m[1].i;  // H01
+    double br = H.m[2].r, bi = H.m[2].i;  // H02
+    double cr = H.m[5].r, ci = H.m[5].i;  // H12
+    double a2 = ar * ar + ai * ai, b2 = br * br + bi * bi, c2 = cr * cr + ci * ci;
+    c1 = 0.5 * (h00 * h00 + h11 * h11 + h22 * h22) + a2 + b2 + c2;
+    // det = h00 h11 h22 - h00|c|^2 - h11|b|^2 - h22|a|^2 + 2 Re(a c conj(b))
+    double re_acb = (ar * cr - ai * ci) * br + (ar * ci + ai * cr) * bi;
+    c0 = h00 * h11 * h22 - h00 * c2 - h11 * b2 - h22 * a2 + 2.0 * re_acb;
+}
+
+// f_j (and, if WANT_B, b_1j = df_j/dc1, b_2j = df_j/dc0) for exp(iH) = f0 + f1 H + f2 H^2.
+// The c0 -> -c0 symmetry keeps 9u^2 - w^2 >= 2 c1; sin(w)/w and cos(w)/w^2 - sin(w)/w^3
+// switch to their Taylor series for |w| < 0.05.  Requires c1 > 0.
+template <bool WANT_B>
+__device__ __forceinline__ void ch_coeffs(double c0, double c1, dc f[3], dc b1[3], dc b2[3]) {
+    const bool neg = c0 < 0.0;
+    const double c0a = fabs(c0);
+    const double c13 = c1 * (1.0 / 3.0);
+    const double c0max = 2.0 * c13 * sqrt(c13);
+    double ratio = c0a / c0max;
+    ratio = ratio > 1.0 ? 1.0 : ratio;
+    const double th = acos(ratio) * (1.0 / 3.0);
+    double sth, cth;
+    sincos(th, &sth, &cth);
+    const double u = sqrt(c13) * cth;
+    const double w = sqrt(c1) * sth;
+    const double u2 = u * u, w2 = w * w;
+    double xi0, xi1 = 0.0;
+    double sw, cw;
+    sincos(w, &sw, &cw);
+    if (fabs(w) < 0.05) {
+        xi0 = 1.0 - w2 * (1.0 / 6.0) * (1.0 - w2 * (1.0 / 20.0) * (1.0 - w2 * (1.0 / 42.0)));
+        if (WANT_B) xi1 = -1.0 / 3.0 + w2 * (1.0 / 30.0) * (1.0 - w2 * (1.0 / 28.0) * (1.0 - w2 * (1.0 / 54.0)));
+    } else {
+        xi0 = sw / w;
+        if (WANT_B) xi1 = cw / w2 - sw / (w2 * w);
+    }
+    double su, cu, s2u, c2u;
+    sincos(u, &su, &cu);
+    sincos(2.0 * u, &s2u, &c2u);
+    const dc emiu = {cu, -su}, e2iu = {c2u, s2u};
+    // h0 = (u^2 - w^2) e2iu + emiu (8u^2 cos w + 2iu(3u^2 + w^2) xi0)
+    dc h0 = dca(dcs(u2 - w2, e2iu), dcm(emiu, dc{8.0 * u2 * cw, 2.0 * u * (3.0 * u2 + w2) * xi0}));
+    // h1 = 2u e2iu - emiu (2u cos w - i(3u^2 - w^2) xi0)
+    dc h1 = dcsub(dcs(2.0 * u, e2iu), dcm(emiu, dc{2.0 * u * cw, -(3.0 * u2 - w2) * xi0}));
+    // h2 = e2iu - emiu (cos w + 3iu xi0)
+    dc h2 = dcsub(e2iu, dcm(emiu, dc{cw, 3.0 * u * xi0}));
+    const double iden = 1.0 / (9.0 * u2 - w2);
+    f[0] = dcs(iden, h0);
+    f[1] = dcs(iden, h1);
+    f[2] = dcs(iden, h2);
+    if (WANT_B) {
+        dc r1[3], r2[3];
+        // r10 = 2(u + i(u^2 - w^2)) e2iu + 2 emiu (4u(2 - iu) cos w + i(9u^2 + w^2 - iu(3u^2 + w^2)) xi0)
+        {
+            dc t1 = dcm(dc{2.0 * u, 2.0 * (u2 - w2)}, e2iu);
+            dc a = {8.0 * u * cw, -4.0 * u2 * cw};                       // 4u(2 - iu) cos w
+            dc bb = {u * (3.0 * u2 + w2) * xi0, (9.0 * u2 + w2) * xi0};  // i(9u^2+w^2 - iu(3u^2+w^2)) xi0
+            r1[0] = dca(t1, dcs(2.0, dcm(emiu, dca(a, bb))));
+        }
+        // r11 = 2(1 + 2iu) e2iu + emiu (-2(1 - iu) cos w + i(6u + i(w^2 - 3u^2)) xi0)
+        {
+            dc t1 = dcm(dc{2.0, 4.0 * u}, e2iu);
+            dc a = {-2.0 * cw, 2.0 * u * cw};
+            dc bb = {-(w2 - 3.0 * u2) * xi0, 6.0 * u * xi0};
+            r1[1] = dca(t1, dcm(emiu, dca(a, bb)));
+        }
+        // r12 = 2i e2iu + i emiu (cos w - 3(1 - iu) xi0)
+        {
+            dc t1 = dcm(dc{0.0, 2.0}, e2iu);
+            dc in = {cw - 3.0 * xi0, 3.0 * u * xi0};
+            r1[2] = dca(t1, dcm(dc{0.0, 1.0}, dcm(emiu, in)));
+        }
+        // r20 = -2 e2iu + 2iu emiu (cos w + (1 + 4iu) xi0 + 3u^2 xi1)
+        {
+            dc in = {cw + xi0 + 3.0 * u2 * xi1, 4.0 * u * xi0};
+            r2[0] = dca(dcs(-2.0, e2iu), dcm(dc{0.0, 2.0 * u}, dcm(emiu, in)));
+        }
+        // r21 = -i emiu (cos w + (1 + 2iu) xi0 - 3u^2 xi1)
+        {
+            dc in = {cw + xi0 - 3.0 * u2 * xi1, 2.0 * u * xi0};
+            r2[1] = dcm(dc{0.0, -1.0}, dcm(emiu, in));
+        }
+        // r22 = emiu (xi0 - 3iu xi1)
+        r2[2] = dcm(emiu, dc{xi0, -3.0 * u * xi1});
+        const double id2 = 0.5 * iden * iden;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            b1[j] = dcs(id2, dcsub(dca(dcs(2.0 * u, r1[j]), dcs(3.0 * u2 - w2, r2[j])),
+                                   dcs(2.0 * (15.0 * u2 + w2), f[j])));
+            b2[j] = dcs(id2, dcsub(dcsub(r1[j], dcs(3.0 * u, r2[j])), dcs(24.0 * u, f[j])));
+        }
+    }
+    if (neg) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const double sg = (j & 1) ? -1.0 : 1.0;
+            f[j] = dcs(sg, dcconj(f[j]));
+            if (WANT_B) {
+                b1[j] = dcs(sg, dcconj(b1[j]));
+                b2[j] = dcs(-sg, dcconj(b2[j]));
+            }
+        }
+    }
+}
+
+// c1 below which exp(iH) = 1 + iH - H^2/2 (error < 1e-30)
+#define GF_CH_EXP_TINY 1e-20
+// c1 below which C star dexp(Q)/dQ is summed from its power series
+#define GF_CH_DER_SERIES 1e-2
+#define GF_CH_DER_TERMS 12
+
+// exp(iH) for Hermitian traceless H
+template <typename T>
+__device__ __forceinline__ M3<T> expi_herm(const M3<T>& H) {
+    double c0, c1;
+    herm_invariants(H, c0, c1);
+    M3<T> H2 = mul(H, H);
+    M3<T> E;
+    if (c1 < GF_CH_EXP_TINY) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            E.m[k].r = -H.m[k].i - T(0.5) * H2.m[k].r;
+            E.m[k].i = H.m[k].r - T(0.5) * H2.m[k].i;
+        }
+        E.m[0].r += T(1);
+        E.m[4].r += T(1);
+        E.m[8].r += T(1);
+        return E;
+    }
+    dc f[3];
+    ch_coeffs<false>(c0, c1, f, nullptr, nullptr);
+    const T f0r = T(f[0].r), f0i = T(f[0].i), f1r = T(f[1].r), f1i = T(f[1].i);
+    const T f2r = T(f[2].r), f2i = T(f[2].i);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        // f1 H + f2 H^2
+        T er = f1r * H.m[k].r - f1i * H.m[k].i + f2r * H2.m[k].r - f2i * H2.m[k].i;
+        T ei = f1r * H.m[k].i + f1i * H.m[k].r + f2r * H2.m[k].i + f2i * H2.m[k].r;
+        E.m[k].r = er;
+        E.m[k].i = ei;
+    }
+    E.m[0].r += f0r;
+    E.m[0].i += f0i;
+    E.m[4].r += f0r;
+    E.m[4].i += f0i;
+    E.m[8].r += f0r;
+    E.m[8].i += f0i;
+    return E;
+}
+
+// exp(Q) for anti-Hermitian traceless Q  (H = -iQ)
+template <typename T>
+__device__ __forceinline__ M3<T> exp_antiherm(const M3<T>& Q) {
+    M3<T> H;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        H.m[k].r = Q.m[k].i;
+        H.m[k].i = -Q.m[k].r;
+    }
+    return expi_herm(H);
+}
+
+// L = C star d exp(Q)/dQ:  d tr(C e^Q) = tr(L dQ) for anti-Hermitian traceless Q
+// (CdexpQdQ!, stout_fast.jl:888-946, 1031-1081)
+template <typename T>
+__device__ __forceinline__ M3<T> cdexpqdq(const M3<T>& C, const M3<T>& Q) {
+    M3<T> H;  // H = -iQ (Hermitian)
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        H.m[k].r = Q.m[k].i;
+        H.m[k].i = -Q.m[k].r;
+    }
+    double c0, c1;
+    herm_invariants(H, c0, c1);
+    if (c1 < GF_CH_DER_SERIES) {
+        // L = sum_{n>=1} 1/n! sum_{k<n} Q^(n-1-k) C Q^k ;  Ln+1 = Q Ln + C Q^n
+        M3<T> L = C, Ln = C, CQn = C;
+        T fact = T(1);
+#pragma unroll 1
+        for (int n = 2; n <= GF_CH_DER_TERMS; n++) {
+            CQn = mul(CQn, Q);
+            M3<T> t = mul(Q, Ln);
+            acc(t, CQn);
+            Ln = t;
+            fact *= T(n);
+            T inv = T(1) / fact;
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                L.m[k].r = fma(inv, Ln.m[k].r, L.m[k].r);
+                L.m[k].i = fma(inv, Ln.m[k].i, L.m[k].i);
+            }
+        }
+        return L;
+    }
+    dc f[3], b1[3], b2[3];
+    ch_coeffs<true>(c0, c1, f, b1, b2);
+    M3<T> H2 = mul(H, H);
+    // traces tr(C), tr(C H), tr(C H^2) in double
+    dc t0 = {0.0, 0.0}, t1 = {0.0, 0.0}, t2 = {0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        t0.r += double(C.m[4 * i].r);
+        t0.i += double(C.m[4 * i].i);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const cx<T> c = C.m[3 * i + k];
+            const cx<T> h = H.m[3 * k + i];
+            const cx<T> g = H2.m[3 * k + i];
+            t1.r += double(c.r) * double(h.r) - double(c.i) * double(h.i);
+            t1.i += double(c.r) * double(h.i) + double(c.i) * double(h.r);
+            t2.r += double(c.r) * double(g.r) - double(c.i) * double(g.i);
+            t2.i += double(c.r) * double(g.i) + double(c.i) * double(g.r);
+        }
+    }
+    const dc tB1 = dca(dca(dcm(b1[0], t0), dcm(b1[1], t1)), dcm(b1[2], t2));
+    const dc tB2 = dca(dca(dcm(b2[0], t0), dcm(b2[1], t1)), dcm(b2[2], t2));
+    const T a1r = T(tB1.r), a1i = T(tB1.i), a2r = T(tB2.r), a2i = T(tB2.i);
+    const T f1r = T(f[1].r), f1i = T(f[1].i), f2r = T(f[2].r), f2i = T(f[2].i);
+    M3<T> HC = mul(H, C);
+    acc_mul(HC, C, H);  // H C + C H
+    M3<T> Lh;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        T r = a1r * H.m[k].r - a1i * H.m[k].i + a2r * H2.m[k].r - a2i * H2.m[k].i + f1r * C.m[k].r -
+              f1i * C.m[k].i + f2r * HC.m[k].r - f2i * HC.m[k].i;
+        T im = a1r * H.m[k].i + a1i * H.m[k].r + a2r * H2.m[k].i + a2i * H2.m[k].r + f1r * C.m[k].i +
+               f1i * C.m[k].r + f2r * HC.m[k].i + f2i * HC.m[k].r;
+        // divide by i: (r + i im)/i = im - i r
+        Lh.m[k].r = im;
+        Lh.m[k].i = -r;
+    }
+    return Lh;
+}
+
+}  // namespace gf

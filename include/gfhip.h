@@ -1,0 +1,135 @@
+/*
+ * gfhip.h -- C ABI of libgfhip.so, the MI355X (gfx950) backend for Gaugefields.jl's SU(3)
+ * gauge-link hot path.
+ *
+ * One gf_ctx holds one 4D periodic SU(3) gauge configuration (the reference's
+ * Vector of four Gaugefields_4D fields) together with its conjugate momenta, force and
+ * work fields, all resident in HBM in a structure-of-arrays layout.  Host arrays cross the
+ * boundary in the reference's own Julia layouts (FP64, interior sites only):
+ *   links   : double[4][V][9][2]  direction mu, site s = x + NX*(y + NY*(z + NZ*t)),
+ *             element (i,j) of the 3x3 matrix at complex offset i + 3*j (column-major),
+ *             real part first -- Array{ComplexF64}(3,3,NX,NY,NZ,NT) per direction
+ *             (gaugefields_4D_nowing.jl:18-51; LatticeMatrix interior,
+ *             gaugefields_4D_MPILattice.jl:34-121).
+ *   momenta : double[4][V][8]     coefficients p_a of sum_a p_a (i lambda_a / 2), Gell-Mann
+ *             order of TA_gaugefields_4D_serial.jl:243-260 (TA_Gaugefields_4D_serial a[k,...]).
+ * Host buffers are borrowed for the duration of the call only.
+ *
+ * Every call returns GF_OK (0) or a negative status; gf_last_error() describes the last
+ * failure.  A context owns one HIP stream; calls on it are stream-ordered and not
+ * thread-safe.  Calls that return host scalars synchronize the stream.
+ *
+ * Precision: GF_F64 stores and computes links/momenta in double.  GF_F32 ("FP32 bulk")
+ * stores and computes them in float while every global sum (plaquette, action, p*p,
+ * Hamiltonian) and every Cayley-Hamilton coefficient is evaluated in double.
+ */
+#ifndef GFHIP_H
+#define GFHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gf_ctx gf_ctx;
+
+enum { GF_F64 = 0, GF_F32 = 1 };
+enum { GF_OK = 0, GF_EINVAL = -1, GF_EDEVICE = -2, GF_ENOMEM = -3, GF_ESTATE = -4 };
+enum { GF_QPQ = 0, GF_PQP = 1, GF_OMELYAN2MN = 2, GF_FUSED = 0x100 };
+enum { GF_STOUT_STORE = 0, GF_STOUT_RECOMPUTE = 1 };
+
+/* ---- lifecycle ------------------------------------------------------------------- */
+int gf_version(void);
+/* gauge_configuration (API.jl:178-251) minus the random start: allocates a configuration
+ * of extents dims[4] = {NX,NY,NZ,NT} on HIP device `device`, links set to the identity. */
+int gf_create(const int *dims, int precision, int device, gf_ctx **out);
+int gf_destroy(gf_ctx *ctx);
+const char *gf_last_error(const gf_ctx *ctx);
+int gf_sync(gf_ctx *ctx);
+/* extents, precision and device bytes held by the context */
+int gf_info(const gf_ctx *ctx, int *dims, int *precision, size_t *device_bytes);
+
+/* ---- fields ---------------------------------------------------------------------- */
+/* cold start: IdentityGauges (AbstractGaugefields.jl:431-548 condition="cold") */
+int gf_set_cold(gf_ctx *ctx);
+/* substitute_U!(U, host) / copy back (gaugefields_4D_MPILattice.jl substitute_U!) */
+int gf_upload_links(gf_ctx *ctx, const double *ref);
+int gf_download_links(gf_ctx *ctx, double *ref);
+/* momenta P (TA_gaugefields_4D_MPILattice.jl:4-72 storage) */
+int gf_upload_momenta(gf_ctx *ctx, const double *mom);
+int gf_download_momenta(gf_ctx *ctx, double *mom);
+/* clear_U!(P) (TA_gaugefields_4D_MPILattice.jl clear_U!) */
+int gf_clear_momenta(gf_ctx *ctx);
+/* the force field F written by gf_force / gf_stout_force (md_force! output) */
+int gf_download_force(gf_ctx *ctx, double *force);
+/* substitute_U!(Uold, U) / substitute_U!(U, Uold): the Metropolis backup slot
+ * (test/HMCstout_test_nowing.jl:47-74) */
+int gf_save_links(gf_ctx *ctx);
+int gf_restore_links(gf_ctx *ctx);
+
+/* ---- measurements ---------------------------------------------------------------- */
+/* calculate_Plaquette (AbstractGaugefields.jl:2684-2699): sum_{x,mu<nu} Re tr P_munu.
+ * measure_plaquette (API.jl:395-404) divides by 6*V*3. */
+int gf_plaquette(gf_ctx *ctx, double *sum);
+/* real(evaluate_GaugeAction) (GaugeActions.jl:132-142) of push!(action, coeff,
+ * plaquette ∪ plaquette^dag): 2*coeff*sum Re tr P.  md_potential = -value/3. */
+int gf_action(gf_ctx *ctx, double coeff, double *value);
+/* p*p (TA_gaugefields_4D_MPILattice.jl:147-155): sum of all p_a^2 */
+int gf_momenta_dot(gf_ctx *ctx, double *value);
+/* md_hamiltonian (molecular_dynamics.jl:494-505): md_potential + p*p/2, on the stout-smeared
+ * links when gf_stout_config set nlev > 0 */
+int gf_hamiltonian(gf_ctx *ctx, double coeff, double *H);
+
+/* ---- derivative, force, MD updates ------------------------------------------------ */
+/* calc_dSdUμ! (GaugeActions.jl:95-123) for all four mu; written to ref (link layout) if
+ * ref != NULL */
+int gf_dsdu(gf_ctx *ctx, double coeff, double *ref);
+/* md_force!(F, ::GaugeAction, U) (molecular_dynamics.jl:251-267): F = -(1/3) TA(U dSdU) */
+int gf_force(gf_ctx *ctx, double coeff);
+/* update_momenta! (molecular_dynamics.jl:539-551): P += eps * F(U), fused in one kernel */
+int gf_force_kick(gf_ctx *ctx, double coeff, double eps);
+/* update_gaugefields! (molecular_dynamics.jl:513-531): U <- exp(eps P) U */
+int gf_exp_update(gf_ctx *ctx, double eps);
+/* update_momenta!(eps_p) followed by update_gaugefields!(eps_u) in one kernel
+ * (force + kick + exp; the new links go to a second buffer that becomes current) */
+int gf_md_update(gf_ctx *ctx, double coeff, double eps_p, double eps_u);
+/* md_trajectory! (molecular_dynamics.jl:712-730) with md_step! QPQ/PQP (:604-616) or the
+ * Omelyan 2MN step of docs/src/howtoimplement.md:94-112 (integrator | GF_FUSED merges
+ * adjacent half steps and uses gf_md_update).  Force: plain Wilson action, or the
+ * stout-smeared one when gf_stout_config set nlev > 0.  H[3] = {H0, H1, H1 - H0}. */
+int gf_md_trajectory(gf_ctx *ctx, double coeff, int integrator, double lambda, int steps, double tau,
+                     double *H);
+
+/* ---- Wilson flow ------------------------------------------------------------------ */
+/* flow!(U, ::Gradientflow) (gradientflow.jl:171-238): nsteps Luscher RK3 steps of size eps.
+ * If every > 0, E_out[k] = energy density after step (k+1)*every. */
+int gf_flow(gf_ctx *ctx, double eps, int nsteps, int every, double *E_out);
+/* plaquette energy density E = 2 sum_{mu<nu} Re tr(1 - P_munu) / V = 36 (1 - plaquette)
+ * (Luscher JHEP 08 (2010) 071; no built-in counterpart in the reference) */
+int gf_energy_density(gf_ctx *ctx, double *E);
+
+/* ---- stout smearing ---------------------------------------------------------------- */
+/* nlev stout layers at rho (STOUT_Layer + CovNeuralnet, stout_fast.jl:6-122,
+ * Abstractsmearing.jl:61-99); mode GF_STOUT_STORE keeps C_mu per level, GF_STOUT_RECOMPUTE
+ * recomputes it in the backward pass.  nlev = 0 disables smearing. */
+int gf_stout_config(gf_ctx *ctx, int nlev, double rho, int mode);
+/* calc_smearedU (Abstractsmearing.jl:237-272): smeared links (all levels kept on device);
+ * top level written to ref if ref != NULL */
+int gf_stout_smear(gf_ctx *ctx, double *ref);
+/* back_prop (Abstractsmearing.jl:352-411): given dS/dU_smeared (link layout) return
+ * dS/dU_bare; uses the levels of the last gf_stout_smear */
+int gf_stout_backward(gf_ctx *ctx, const double *dout_ref, double *din_ref);
+/* stout MD force (test/HMCstout_test_nowing.jl:99-118): smear, calc_dSdUμ! on the smeared
+ * links, back_prop, F = -(1/3) TA(U dSdU_bare) */
+int gf_stout_force(gf_ctx *ctx, double coeff);
+int gf_stout_force_kick(gf_ctx *ctx, double coeff, double eps);
+
+/* ---- timing (HIP events on the context's stream) ----------------------------------- */
+int gf_event_record(gf_ctx *ctx, int slot); /* slot 0..15 */
+int gf_event_elapsed(gf_ctx *ctx, int slot_a, int slot_b, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,260 @@
+"""GPU parity: libgfhip (through its C ABI) against the CPU oracle on identical seeded inputs.
+
+FP64 tolerances follow the stated bar of relative 1e-12 for plaquette, action, force and
+Hamiltonian (for dH, relative to H: the Hamiltonian is summed over the lattice and its
+difference inherits the absolute rounding of the sum).  FP32-bulk results are compared
+against the FP64 oracle with the tolerances written in each test.
+"""
+import numpy as np
+import pytest
+
+import lattice_tools as lt
+
+pytestmark = pytest.mark.gpu
+
+gen = lt.gen
+D4 = (4, 4, 4, 4)
+COEFF = 3.0  # beta = 6
+
+
+@pytest.fixture(scope="module")
+def Lat(gf):
+    made = []
+
+    def make(dims=D4, precision="f64"):
+        L = gf.Lattice(dims, precision=precision)
+        made.append(L)
+        return L
+
+    yield make
+    for L in made:
+        L.close()
+
+
+def alg_links(oracle, dims, sigma, seed):
+    U = gen.cold_links(dims)
+    oracle.exp_update(dims, U, gen.algebra_coefficients(dims, sigma, seed), 1.0)
+    return U
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def maxdiff(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max())
+
+
+# ------------------------------------------------------------------ layout, plaquette
+
+
+def test_roundtrip_exact(Lat):
+    L = Lat()
+    U = gen.hot_start_links(D4, 1)
+    P = gen.gaussian_momenta(D4, 2)
+    L.upload_links(U)
+    L.upload_momenta(P)
+    assert np.array_equal(L.download_links(), U)
+    assert np.array_equal(L.download_momenta(), P)
+
+
+@pytest.mark.parametrize("dims", [D4, (2, 2, 2, 2), (4, 2, 3, 2), (2, 4, 2, 6), (8, 4, 6, 2)])
+def test_plaquette_parity(Lat, oracle, dims):
+    L = Lat(dims)
+    assert L.plaquette() == 1.0  # cold start
+    U = gen.hot_start_links(dims, 5)
+    L.upload_links(U)
+    assert rel(L.plaquette_sum(), oracle.plaquette_sum(dims, U)) < 1e-12
+    assert rel(L.action(2.75), 2 * 2.75 * oracle.plaquette_sum(dims, U)) < 1e-12
+
+
+def test_instanton_golden_gpu(Lat):
+    # test/init.jl:351-371 (4x4x4x1 lattice) embedded in SU(3): (2*0.9796864531099871 + 1)/3
+    dims = (4, 4, 4, 1)
+    L = Lat(dims)
+    L.upload_links(gen.matrices_to_links(lt.instanton_links(dims, nc=3)))
+    assert abs(L.plaquette() - (2 * 0.9796864531099871 + 1) / 3) < 1e-14
+
+
+def test_plaquette_16_gauge_invariant(Lat, oracle):
+    dims = (16, 16, 16, 16)
+    L = Lat(dims)
+    U = alg_links(oracle, dims, 0.3, 8)
+    L.upload_links(U)
+    p = L.plaquette_sum()
+    assert rel(p, oracle.plaquette_sum(dims, U)) < 1e-12
+    g = lt.random_su3(np.prod(dims), np.random.default_rng(1))
+    L.upload_links(lt.gauge_transform(U, dims, g))
+    assert rel(L.plaquette_sum(), p) < 1e-12
+
+
+# ------------------------------------------------------------------ derivative, force, exp
+
+
+@pytest.mark.parametrize("dims", [D4, (4, 2, 3, 2)])
+def test_dsdu_parity(Lat, oracle, dims):
+    L = Lat(dims)
+    U = gen.hot_start_links(dims, 11)
+    L.upload_links(U)
+    assert maxdiff(L.dsdu(COEFF), oracle.dsdu(dims, U, COEFF)) < 1e-13 * COEFF * 6
+
+
+@pytest.mark.parametrize("dims", [D4, (2, 4, 2, 6)])
+def test_force_parity(Lat, oracle, dims):
+    L = Lat(dims)
+    U = alg_links(oracle, dims, 0.3, 12)
+    L.upload_links(U)
+    L.force(COEFF)
+    F = oracle.force(dims, U, COEFF)
+    assert maxdiff(L.download_force(), F) < 1e-12 * np.abs(F).max()
+
+
+def test_kick_and_exp_parity(Lat, oracle):
+    L = Lat()
+    U = alg_links(oracle, D4, 0.3, 13)
+    P = gen.gaussian_momenta(D4, 14)
+    L.upload_links(U)
+    L.upload_momenta(P)
+    L.force_kick(COEFF, 0.02)
+    P2 = P.copy()
+    oracle.kick(D4, P2, oracle.force(D4, U, COEFF), 0.02)
+    assert maxdiff(L.download_momenta(), P2) < 1e-14
+    L.exp_update(0.02)
+    U2 = U.copy()
+    oracle.exp_update(D4, U2, P2, 0.02)
+    assert maxdiff(L.download_links(), U2) < 1e-14
+
+
+def test_fused_md_update(Lat, oracle):
+    L = Lat()
+    U = alg_links(oracle, D4, 0.3, 15)
+    P = gen.gaussian_momenta(D4, 16)
+    L.upload_links(U)
+    L.upload_momenta(P)
+    L.md_update(COEFF, 0.03, 0.02)
+    U2, P2 = U.copy(), P.copy()
+    oracle.kick(D4, P2, oracle.force(D4, U, COEFF), 0.03)
+    oracle.exp_update(D4, U2, P2, 0.02)
+    assert maxdiff(L.download_momenta(), P2) < 1e-14
+    assert maxdiff(L.download_links(), U2) < 1e-14
+
+
+def test_exp_large_and_tiny_momenta(Lat, oracle):
+    L = Lat()
+    for scale in (3.0, 1e-9, 0.0):
+        P = scale * gen.gaussian_momenta(D4, 17)
+        L.set_cold()
+        L.upload_momenta(P)
+        L.exp_update(1.0)
+        U = gen.cold_links(D4)
+        oracle.exp_update(D4, U, P, 1.0)
+        assert maxdiff(L.download_links(), U) < 5e-15 * max(1.0, scale)
+
+
+# ------------------------------------------------------------------ MD
+
+
+@pytest.mark.parametrize("integ", ["QPQ", "PQP", "2MN"])
+@pytest.mark.parametrize("fused", [False, True])
+def test_md_trajectory_parity(Lat, oracle, integ, fused):
+    L = Lat()
+    U = gen.hot_start_links(D4, 21)
+    P = gen.gaussian_momenta(D4, 22)
+    L.upload_links(U)
+    L.upload_momenta(P)
+    H = L.md_trajectory(COEFF, 10, 1.0, integ, fused=fused)
+    Href = oracle.md_trajectory(D4, U, P, COEFF, 10, 1.0, integ)
+    assert rel(H[0], Href[0]) < 1e-12 and rel(H[1], Href[1]) < 1e-12
+    assert abs(H[2] - Href[2]) < 1e-12 * abs(Href[0])
+    assert maxdiff(L.download_links(), U) < 1e-11
+    assert maxdiff(L.download_momenta(), P) < 1e-10
+
+
+def test_md_cold_start_gpu(Lat):
+    L = Lat()
+    H = L.md_trajectory(COEFF, 5, 1.0, "QPQ")
+    assert H[2] == 0.0 and L.plaquette() == 1.0
+
+
+def test_md_reversibility_gpu(Lat):
+    L = Lat()
+    U0 = gen.hot_start_links(D4, 23)
+    P0 = gen.gaussian_momenta(D4, 24)
+    L.upload_links(U0)
+    L.upload_momenta(P0)
+    L.md_trajectory(COEFF, 5, 0.2, "2MN", fused=True)
+    L.md_trajectory(COEFF, 5, -0.2, "2MN", fused=True)
+    assert maxdiff(L.download_links(), U0) < 2e-12
+    assert maxdiff(L.download_momenta(), P0) < 2e-12
+
+
+# ------------------------------------------------------------------ flow
+
+
+def test_flow_parity(Lat, oracle):
+    dims = (4, 4, 4, 6)
+    L = Lat(dims)
+    U = alg_links(oracle, dims, 0.4, 31)
+    L.upload_links(U)
+    E = L.flow(0.01, 20, 5)
+    Eref = oracle.flow(dims, U, 0.01, 20, 5)
+    assert np.all(np.abs(E - Eref) < 1e-12 * np.abs(Eref))
+    assert maxdiff(L.download_links(), U) < 1e-13
+
+
+# ------------------------------------------------------------------ stout
+
+
+@pytest.mark.parametrize("mode", ["store", "recompute"])
+def test_stout_smear_and_force_parity(Lat, oracle, mode):
+    L = Lat()
+    U = alg_links(oracle, D4, 0.3, 41)
+    L.upload_links(U)
+    L.stout_config(3, 0.1, mode)
+    assert maxdiff(L.stout_smear(), oracle.stout_smear(D4, U, 3, 0.1)) < 1e-14
+    L.force(COEFF)
+    F = oracle.stout_force(D4, U, 3, 0.1, COEFF)
+    assert maxdiff(L.download_force(), F) < 1e-12 * np.abs(F).max()
+    H = L.hamiltonian(COEFF)
+    assert rel(H, oracle.stout_potential(D4, U, 3, 0.1, COEFF)) < 1e-12  # momenta are zero
+
+
+def test_stout_backward_level_parity(Lat, oracle):
+    L = Lat()
+    U = gen.hot_start_links(D4, 42)
+    D = gen.hot_start_links(D4, 43)  # any link-shaped derivative
+    L.upload_links(U)
+    L.stout_config(1, 0.12, "store")
+    L.stout_smear(download=False)
+    assert maxdiff(L.stout_backward(D), oracle.stout_backward(D4, U, 0.12, D)) < 1e-13
+
+
+def test_stout_md_parity(Lat, oracle):
+    L = Lat()
+    U = alg_links(oracle, D4, 0.3, 44)
+    P = gen.gaussian_momenta(D4, 45)
+    L.upload_links(U)
+    L.upload_momenta(P)
+    L.stout_config(3, 0.1, "store")
+    H = L.md_trajectory(COEFF, 4, 0.2, "2MN", fused=True)
+    Href = oracle.md_trajectory(D4, U, P, COEFF, 4, 0.2, "2MN", nlev=3, rho=0.1)
+    assert abs(H[2] - Href[2]) < 1e-12 * abs(Href[0])
+    assert maxdiff(L.download_links(), U) < 1e-12
+
+
+# ------------------------------------------------------------------ FP32 bulk
+
+
+def test_fp32_bulk_against_fp64_oracle(Lat, oracle):
+    L = Lat(precision="f32")
+    U = alg_links(oracle, D4, 0.3, 51)
+    P = gen.gaussian_momenta(D4, 52)
+    L.upload_links(U)
+    assert rel(L.plaquette_sum(), oracle.plaquette_sum(D4, U)) < 2e-7
+    L.force(COEFF)
+    F = oracle.force(D4, U, COEFF)
+    assert maxdiff(L.download_force(), F) < 2e-6 * np.abs(F).max()
+    L.upload_momenta(P)
+    H = L.md_trajectory(COEFF, 10, 1.0, "QPQ", fused=True)
+    Href = oracle.md_trajectory(D4, U.copy(), P.copy(), COEFF, 10, 1.0, "QPQ")
+    assert abs(H[2] - Href[2]) < 1e-3
