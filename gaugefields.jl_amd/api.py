@@ -1,0 +1,532 @@
+"""Host-side mirror of Gaugefields.jl's user API for the SU(3) hot path, on libgfhip.
+
+Names, argument meaning and error behaviour follow the reference (Julia ``!`` functions
+are the same names without ``!``; they mutate their first argument in place):
+
+  gauge_configuration        API.jl:178-251
+  measure_plaquette          API.jl:395-404       calculate_Plaquette  AbstractGaugefields.jl:2684-2699
+  gauge_momenta              API.jl:330-331       gaussian_momenta(!)  API.jl:341-392
+  GaugeAction / push         GaugeActions.jl:25-62, 238-246
+  evaluate_GaugeAction       GaugeActions.jl:132-142
+  calc_dSdUmu                GaugeActions.jl:77-123
+  md_action_workspace / md_potential / md_force   molecular_dynamics.jl:13-43, 243-267
+  md_driver / md_step_size / md_hamiltonian       molecular_dynamics.jl:425-505
+  update_gaugefields / update_momenta             molecular_dynamics.jl:513-551
+  QPQ / PQP / OmelyanPQP / md_step / md_trajectory molecular_dynamics.jl:180-200, 594-730;
+                                                   docs/src/howtoimplement.md:94-112
+  gradient_flow / flow       API.jl:420-437, gradientflow.jl:118-238
+  stout_smearing / smear / back_prop   API.jl:452-480, Abstractsmearing.jl:237-411
+
+Only what the hot path needs exists here: SU(3), four dimensions, periodic boundaries,
+the Wilson plaquette action (loop set plaquette ∪ plaquette^dag).  Everything runs
+through the HIP library; there is no CPU fallback.
+"""
+import math
+
+import numpy as np
+
+from . import generators
+from .lattice import OMELYAN_LAMBDA, Lattice
+
+__all__ = [
+    "ArgumentError",
+    "HIPBackend",
+    "HIPGaugeConfiguration",
+    "HIPMomenta",
+    "gauge_configuration",
+    "measure_plaquette",
+    "calculate_Plaquette",
+    "gauge_momenta",
+    "gaussian_momenta",
+    "gaussian_momenta_",
+    "GaugeAction",
+    "make_loops_fromname",
+    "evaluate_GaugeAction",
+    "calc_dSdUmu",
+    "md_action_workspace",
+    "md_potential",
+    "md_force",
+    "QPQ",
+    "PQP",
+    "OmelyanPQP",
+    "MDDriver",
+    "md_driver",
+    "md_step_size",
+    "md_hamiltonian",
+    "update_gaugefields",
+    "update_momenta",
+    "md_step",
+    "md_trajectory",
+    "Gradientflow",
+    "gradient_flow",
+    "flow",
+    "energy_density",
+    "StoutSmearing",
+    "stout_smearing",
+    "smear",
+    "back_prop",
+    "SmearedGaugeAction",
+]
+
+
+class ArgumentError(ValueError):
+    """Julia's ArgumentError."""
+
+
+class HIPBackend:
+    """Backend selector for gauge_configuration (the reference's AbstractGaugeBackend seam)."""
+
+
+# ----------------------------------------------------------------------------- fields
+
+
+class HIPGaugeConfiguration:
+    """The four link fields U[1..4] of one configuration, resident on the GPU.
+
+    ``len(U) == 4``; ``U.NC == 3``; ``U.NV`` sites.  The momenta of the configuration share its
+    device context (``gauge_momenta(U)``)."""
+
+    NC = 3
+    Dim = 4
+
+    def __init__(self, lattice: Lattice):
+        self.lattice = lattice
+        self.NV = lattice.V
+        self.dims = lattice.dims
+        self.smearing = None
+
+    def __len__(self):
+        return 4
+
+    @property
+    def eltype(self):
+        return np.complex128 if self.lattice.precision == "f64" else np.complex64
+
+    def to_host(self):
+        """links as (4, V, 9) complex128, reference layout"""
+        return self.lattice.download_links()
+
+    def from_host(self, links):
+        self.lattice.upload_links(links)
+
+    def copy_to_backup(self):
+        """substitute_U!(Uold, U)"""
+        self.lattice.save_links()
+
+    def restore_from_backup(self):
+        """substitute_U!(U, Uold)"""
+        self.lattice.restore_links()
+
+
+class HIPMomenta:
+    """Conjugate momenta (8 su(3) coefficients per link) living in the configuration's context."""
+
+    def __init__(self, U: HIPGaugeConfiguration):
+        self.U = U
+        self.lattice = U.lattice
+
+    def __len__(self):
+        return 4
+
+    def to_host(self):
+        return self.lattice.download_momenta()
+
+    def from_host(self, P):
+        self.lattice.upload_momenta(P)
+
+    def __mul__(self, other):
+        """p*p (TA_gaugefields_4D_MPILattice.jl:147-155)"""
+        if other is not self:
+            raise ArgumentError("p*q is only provided for q === p on the HIP backend")
+        return self.lattice.momenta_dot()
+
+
+def _check_same(U, P):
+    if not isinstance(P, HIPMomenta) or P.lattice is not U.lattice:
+        raise ArgumentError("U and P must belong to the same HIP configuration")
+
+
+def gauge_configuration(lattice, *, backend=None, colors=3, halo=1, start="cold", seed=None,
+                        eltype=np.complex128, boundary="periodic", verbose=0, device=0):
+    """Create a configuration on the GPU (gauge_configuration, API.jl:178-251).
+
+    start="cold": unit links.  start="hot": each link a seeded complex-Gaussian matrix
+    projected to SU(3) (see generators.hot_start_links); ``seed=None`` draws a fresh seed."""
+    if backend is not None and not isinstance(backend, HIPBackend):
+        raise ArgumentError("only HIPBackend() is available here")
+    dims = tuple(int(d) for d in lattice)
+    if len(dims) != 4:
+        raise ArgumentError(f"the HIP backend is four dimensional; got {len(dims)} extents")
+    if any(d <= 0 for d in dims):
+        raise ArgumentError(f"lattice extents must be positive; got {dims}")
+    if colors != 3:
+        raise ArgumentError(f"the HIP backend implements SU(3) only; got colors={colors}")
+    if start not in ("cold", "hot"):
+        raise ArgumentError(f"start must be :cold or :hot; got {start}")
+    if int(halo) < 0:
+        raise ArgumentError(f"halo must be nonnegative; got {halo}")
+    if boundary != "periodic":
+        raise ArgumentError("the HIP backend supports periodic boundaries only")
+    if eltype not in (np.complex128, np.complex64):
+        raise ArgumentError("eltype must be ComplexF64 (np.complex128) or ComplexF32 (np.complex64)")
+    L = Lattice(dims, precision="f64" if eltype == np.complex128 else "f32", device=device)
+    U = HIPGaugeConfiguration(L)
+    if start == "hot":
+        if seed is None:
+            seed = int(np.random.SeedSequence().entropy % (2 ** 63))
+        L.upload_links(generators.hot_start_links(dims, seed))
+    return U
+
+
+def calculate_Plaquette(U):
+    """sum_{x, mu<nu} Re tr P_munu (the unnormalized historical value)"""
+    return U.lattice.plaquette_sum()
+
+
+def measure_plaquette(U, *, normalize=True):
+    v = calculate_Plaquette(U)
+    return v / (6 * U.NV * 3) if normalize else v
+
+
+def gauge_momenta(U):
+    """zero-valued momenta compatible with U"""
+    P = HIPMomenta(U)
+    U.lattice.clear_momenta()
+    return P
+
+
+def gaussian_momenta_(momenta, *, sigma=1.0, seed=None, sweep=0):
+    """Fill momenta with N(0, sigma^2) coefficients (gaussian_momenta!, API.jl:341-368).
+    The stream is numpy Philox(seed, sweep), not the reference's LatticeMatrices stream."""
+    if sweep < 0:
+        raise ArgumentError(f"sweep must be nonnegative; got {sweep}")
+    if seed is None:
+        seed = int(np.random.SeedSequence().entropy % (2 ** 63))
+    dims = momenta.U.dims
+    P = generators.gaussian_momenta(dims, int(seed) * 1000003 + int(sweep), sigma=sigma)
+    momenta.lattice.upload_momenta(P)
+    return momenta
+
+
+def gaussian_momenta(U, *, sigma=1.0, seed=None, sweep=0):
+    return gaussian_momenta_(HIPMomenta(U), sigma=sigma, seed=seed, sweep=sweep)
+
+
+# ----------------------------------------------------------------------------- actions
+
+
+_PLAQ = "plaquette"
+_PLAQ_DAG = "plaquette'"
+
+
+def make_loops_fromname(name, Dim=4):
+    """Loop-set token for make_loops_fromname("plaquette"); `loops + dagger(loops)` closes it."""
+    if name != "plaquette":
+        raise ArgumentError(f"only the plaquette loop set is implemented on the HIP backend; got {name!r}")
+    if Dim != 4:
+        raise ArgumentError("Dim must be 4")
+    return [_PLAQ]
+
+
+def dagger(loops):
+    """loops' (the Hermitian-conjugate paths)"""
+    return [{_PLAQ: _PLAQ_DAG, _PLAQ_DAG: _PLAQ}[x] for x in loops]
+
+
+class GaugeAction:
+    """Wilson-loop action restricted to coefficient * (plaquette ∪ plaquette^dag) terms."""
+
+    def __init__(self, U, hascovnet=False):
+        self.U = U
+        self.terms = []  # list of coefficients
+
+    def push(self, coefficient, closedloops):
+        loops = sorted(closedloops)
+        if loops != sorted([_PLAQ, _PLAQ_DAG]):
+            raise ArgumentError(
+                "the HIP backend evaluates exactly the loop set plaquette ∪ plaquette' "
+                f"(got {closedloops}); other Wilson loops are not on this path")
+        self.terms.append(complex(coefficient))
+        return self
+
+    @property
+    def coefficient(self):
+        c = sum(self.terms)
+        if c.imag != 0:
+            raise ArgumentError("the HIP backend needs a real action coefficient")
+        return c.real
+
+
+def evaluate_GaugeAction(action, U):
+    """traced, lattice-summed loop expression: 2 * coeff * sum Re tr P"""
+    return U.lattice.action(action.coefficient)
+
+
+def calc_dSdUmu(action, mu, U):
+    """dS/dU_mu (3x3 per site) for mu in 1..4, as (V, 3, 3) complex128 [i, j]"""
+    if mu not in (1, 2, 3, 4):
+        raise ArgumentError(f"mu must be in 1:4; got {mu}")
+    D = U.lattice.dsdu(action.coefficient)
+    return generators.links_to_matrices(D)[mu - 1]
+
+
+class _Workspace:
+    pass
+
+
+def md_action_workspace(action, U):
+    if not isinstance(action, (GaugeAction, SmearedGaugeAction)):
+        raise ArgumentError(f"md_action_workspace is not implemented for {type(action).__name__}")
+    return _Workspace()
+
+
+def md_potential(action, U, workspace=None):
+    if isinstance(action, SmearedGaugeAction):
+        return action.potential(U)
+    if not isinstance(action, GaugeAction):
+        raise ArgumentError(f"md_potential is not implemented for {type(action).__name__}")
+    return -evaluate_GaugeAction(action, U) / U.NC
+
+
+def md_force(force, action, U, workspace=None):
+    """Write dp/dtau into the configuration's force field; returns it as (4, V, 8)."""
+    if isinstance(action, SmearedGaugeAction):
+        action.configure(U)
+    elif isinstance(action, GaugeAction):
+        U.lattice.stout_config(0, 0.0)
+    else:
+        raise ArgumentError(f"md_force! is not implemented for {type(action).__name__}")
+    U.lattice.force(action.coefficient)
+    F = U.lattice.download_force()
+    if force is not None:
+        force[...] = F
+    return F
+
+
+# ----------------------------------------------------------------------------- MD
+
+
+class QPQ:
+    """Q(1/2) P(1) Q(1/2) (molecular_dynamics.jl:190-198)"""
+
+    code = "QPQ"
+
+
+class PQP:
+    """P(1/2) Q(1) P(1/2) (molecular_dynamics.jl:180-188)"""
+
+    code = "PQP"
+
+
+class OmelyanPQP:
+    """P(l) Q(1/2) P(1-2l) Q(1/2) P(l) (docs/src/howtoimplement.md:94-112)"""
+
+    code = "2MN"
+
+    def __init__(self, lam=OMELYAN_LAMBDA):
+        self.lam = float(lam)
+
+
+class MDDriver:
+    def __init__(self, action, integrator, trajectory_length, steps):
+        self.action = action
+        self.integrator = integrator
+        self.trajectory_length = trajectory_length
+        self.steps = steps
+
+
+def md_driver(U, action, *, steps, trajectory_length=1.0, integrator=None):
+    """molecular_dynamics.jl:425-483 validation and construction"""
+    if len(U) != 4:
+        raise ArgumentError(f"the gauge configuration has length {len(U)}; expected 4")
+    if steps <= 0:
+        raise ArgumentError(f"steps must be positive; got {steps}")
+    if not math.isfinite(trajectory_length):
+        raise ArgumentError(f"trajectory_length must be finite; got {trajectory_length}")
+    if trajectory_length == 0:
+        raise ArgumentError("trajectory_length must not be zero")
+    if U.lattice.precision == "f32":
+        tl32 = float(np.float32(trajectory_length))
+        if tl32 == 0:
+            raise ArgumentError("trajectory_length becomes zero after conversion to Float32")
+        trajectory_length = tl32
+    md_action_workspace(action, U)
+    return MDDriver(action, integrator if integrator is not None else QPQ(), trajectory_length, int(steps))
+
+
+def md_step_size(driver):
+    return driver.trajectory_length / driver.steps
+
+
+def _configure(driver, U):
+    a = driver.action
+    if isinstance(a, SmearedGaugeAction):
+        a.configure(U)
+    else:
+        U.lattice.stout_config(0, 0.0)
+    return a.coefficient
+
+
+def md_hamiltonian(U, P, driver):
+    _check_same(U, P)
+    return U.lattice.hamiltonian(_configure(driver, U))
+
+
+def update_gaugefields(U, P, step_size, driver):
+    """U_mu <- exp(step_size P_mu) U_mu"""
+    _check_same(U, P)
+    if not math.isfinite(step_size):
+        raise ArgumentError(f"the gauge-field step size must be finite; got {step_size}")
+    U.lattice.exp_update(step_size)
+    return U
+
+
+def update_momenta(P, U, step_size, driver):
+    """P <- P + step_size * force(U)"""
+    _check_same(U, P)
+    if not math.isfinite(step_size):
+        raise ArgumentError(f"the momentum step size must be finite; got {step_size}")
+    U.lattice.force_kick(_configure(driver, U), step_size)
+    return P
+
+
+def md_step(integrator, U, P, step_size, driver):
+    if callable(integrator) and not isinstance(integrator, (QPQ, PQP, OmelyanPQP)):
+        return integrator(U, P, step_size, driver)
+    if isinstance(integrator, PQP):
+        update_momenta(P, U, step_size / 2, driver)
+        update_gaugefields(U, P, step_size, driver)
+        update_momenta(P, U, step_size / 2, driver)
+    elif isinstance(integrator, QPQ):
+        update_gaugefields(U, P, step_size / 2, driver)
+        update_momenta(P, U, step_size, driver)
+        update_gaugefields(U, P, step_size / 2, driver)
+    elif isinstance(integrator, OmelyanPQP):
+        lam = integrator.lam
+        update_momenta(P, U, lam * step_size, driver)
+        update_gaugefields(U, P, 0.5 * step_size, driver)
+        update_momenta(P, U, (1 - 2 * lam) * step_size, driver)
+        update_gaugefields(U, P, 0.5 * step_size, driver)
+        update_momenta(P, U, lam * step_size, driver)
+    else:
+        raise ArgumentError(f"md_step! is not implemented for {type(integrator).__name__}")
+
+
+def md_trajectory(U, P, driver, *, diagnostics=True, fused=False):
+    """One deterministic MD trajectory (no momentum refresh, no accept/reject).
+
+    Built-in integrators run in the library's native driver (one C call per trajectory;
+    ``fused=True`` merges adjacent half steps into single force + kick + exp kernels).
+    Custom integrators (callables) are stepped from here."""
+    _check_same(U, P)
+    integ = driver.integrator
+    if isinstance(integ, (QPQ, PQP, OmelyanPQP)):
+        coeff = _configure(driver, U)
+        lam = integ.lam if isinstance(integ, OmelyanPQP) else OMELYAN_LAMBDA
+        H = U.lattice.md_trajectory(coeff, driver.steps, driver.trajectory_length, integ.code, lam=lam,
+                                    fused=fused, diagnostics=diagnostics)
+        if not diagnostics:
+            return None
+        return {"initial_hamiltonian": H[0], "final_hamiltonian": H[1], "delta_hamiltonian": H[2]}
+    h0 = md_hamiltonian(U, P, driver) if diagnostics else None
+    eps = md_step_size(driver)
+    for _ in range(driver.steps):
+        md_step(integ, U, P, eps, driver)
+    if not diagnostics:
+        return None
+    h1 = md_hamiltonian(U, P, driver)
+    return {"initial_hamiltonian": h0, "final_hamiltonian": h1, "delta_hamiltonian": h1 - h0}
+
+
+# ----------------------------------------------------------------------------- flow
+
+
+class Gradientflow:
+    def __init__(self, U, Nflow=1, eps=0.01):
+        self.U = U
+        self.Nflow = int(Nflow)
+        self.eps = float(eps)
+
+
+def gradient_flow(U, *, steps=1, step_size=0.01):
+    if steps <= 0:
+        raise ArgumentError(f"steps must be positive; got {steps}")
+    if step_size <= 0:
+        raise ArgumentError(f"step_size must be positive; got {step_size}")
+    return Gradientflow(U, Nflow=steps, eps=step_size)
+
+
+def flow(U, g, *, measure_every=0):
+    """Nflow Luscher RK3 steps in place; with measure_every > 0 returns E(t) samples."""
+    return U.lattice.flow(g.eps, g.Nflow, measure_every)
+
+
+def energy_density(U):
+    """E = 2 sum_{mu<nu} Re tr(1 - P_munu) / V (plaquette definition)"""
+    return U.lattice.energy_density()
+
+
+# ----------------------------------------------------------------------------- stout
+
+
+class StoutSmearing:
+    """A stack of isotropic plaquette stout layers (CovNeuralnet of STOUT_Layer)."""
+
+    def __init__(self, U, levels, rho, mode="store"):
+        if levels < 0:
+            raise ArgumentError("levels must be nonnegative")
+        if mode not in ("store", "recompute"):
+            raise ArgumentError("mode must be 'store' or 'recompute'")
+        self.levels = int(levels)
+        self.rho = float(rho)
+        self.mode = mode
+
+    def configure(self, U):
+        U.lattice.stout_config(self.levels, self.rho, self.mode)
+
+
+def stout_smearing(U, *, loops="plaquette", rho=0.1, levels=1, mode="store"):
+    """stout_smearing (API.jl:452-467): `levels` plaquette layers at the same rho"""
+    names = [loops] if isinstance(loops, str) else list(loops)
+    if names != ["plaquette"]:
+        raise ArgumentError("only plaquette stout layers are implemented on the HIP backend")
+    rhos = [rho] if np.isscalar(rho) else list(rho)
+    if len(rhos) != 1:
+        raise ArgumentError("rho must be a scalar or have one entry per loop")
+    return StoutSmearing(U, levels, rhos[0], mode)
+
+
+def smear(U, smearing, *, record=False):
+    """smeared links (4, V, 9) complex128; record=True also returns the setting"""
+    smearing.configure(U)
+    out = U.lattice.stout_smear() if smearing.levels > 0 else U.to_host()
+    return {"configuration": out, "smearing": smearing} if record else out
+
+
+def back_prop(dSdU_smeared, smearing, U):
+    """dS/dU_bare from dS/dU_smeared (both (4, V, 9) reference layout); re-smears U first"""
+    smearing.configure(U)
+    U.lattice.stout_smear(download=False)
+    return U.lattice.stout_backward(dSdU_smeared)
+
+
+class SmearedGaugeAction:
+    """MD action provider: a GaugeAction evaluated on stout-smeared links
+    (test/HMCstout_test_nowing.jl:36-118)."""
+
+    def __init__(self, action: GaugeAction, smearing: StoutSmearing):
+        self.action = action
+        self.smearing = smearing
+
+    @property
+    def coefficient(self):
+        return self.action.coefficient
+
+    def configure(self, U):
+        self.smearing.configure(U)
+
+    def potential(self, U):
+        self.configure(U)
+        U.lattice.stout_smear(download=False)
+        # potential of the top smeared level with zero momenta contribution
+        return U.lattice.hamiltonian(self.coefficient) - 0.5 * U.lattice.momenta_dot()
