@@ -31,6 +31,35 @@ BETA = 6.0
 COEFF = BETA / 2
 
 
+def kernel_source_sha1():
+    """sha1 over the HIP sources (identifies which kernels a recorded profile belongs to)"""
+    import glob
+    import hashlib
+
+    h = hashlib.sha1()
+    csrc = os.path.join(ROOT, "gaugefields.jl_amd", "csrc")
+    for p in sorted(glob.glob(os.path.join(csrc, "*.hip")) + glob.glob(os.path.join(csrc, "*.hpp"))):
+        h.update(os.path.basename(p).encode())
+        with open(p, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def recorded_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprof FETCH_SIZE/WRITE_SIZE summary,
+    only if it was recorded for the kernel sources built here (else None)"""
+    path = os.path.join(ROOT, "profiles", "traffic_32x32x32x32_f64.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except OSError:
+        return None
+    if rec.get("source_sha1") != kernel_source_sha1():
+        return None
+    k = rec["kernels"].get(kernel)
+    return None if k is None else k["traffic_bytes"]
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -79,24 +108,77 @@ def cpu_baseline(dims, updates):
     }
 
 
-def stout_hmc(gf, precision, ntraj, dims=DIMS):
-    """s/traj of 3-level stout HMC (rho 0.12, 2MN, 16 steps, tau 1) + per-trajectory dH"""
-    L = make_lattice(gf, dims, precision, seed_u=777, seed_p=778)
-    L.stout_config(3, 0.12, "store")
+def flow_compare(gf, dims=DIMS, steps=300, eps=0.01, every=10):
+    """Wilson flow (RK3, eps 0.01, 300 steps) from seeded omega ~ N(0, 0.4^2) links, FP64 vs
+    FP32 bulk: time per flow step and max |t^2 E| difference"""
     gen = gf.generators
-    dH, times = [], []
-    for k in range(ntraj + 1):
-        L.upload_momenta(gen.gaussian_momenta(dims, 1000 + k))
+    omega = gen.algebra_coefficients(dims, 0.4, 4242)
+    out = {}
+    for prec in ("f64", "f32"):
+        L = gf.Lattice(dims, precision=prec)
+        L.links_from_algebra(omega)
         L.sync()
-        t0 = time.perf_counter()
-        H = L.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True)
-        L.sync()
-        if k > 0:  # first trajectory is warm-up
+        L.event_record(4)
+        E = L.flow(eps, steps, every)
+        L.event_record(5)
+        ms = L.event_elapsed_ms(4, 5)
+        t = eps * every * np.arange(1, len(E) + 1)
+        out[prec] = {"ms_per_step": ms / steps, "t2E": t * t * E}
+        L.close()
+    d = np.abs(out["f32"]["t2E"] - out["f64"]["t2E"])
+    return {"lattice": "x".join(map(str, dims)), "eps": eps, "steps": steps,
+            "fp64_ms_per_flow_step": out["f64"]["ms_per_step"],
+            "fp32_ms_per_flow_step": out["f32"]["ms_per_step"],
+            "t2E_final_fp64": float(out["f64"]["t2E"][-1]),
+            "max_abs_t2E_diff_fp32": float(d.max()),
+            "max_rel_t2E_diff_fp32": float((d / np.abs(out["f64"]["t2E"])).max())}
+
+
+def stout_hmc_thermalized(gf, dims=DIMS, ntherm=6):
+    """3-level stout HMC (rho 0.12, 2MN 16 steps, tau 1): FP64 thermalization trajectories
+    (accepted unconditionally), then one trajectory from the same links and momenta in FP64
+    and in FP32 bulk: s/traj, dH agreement, plaquette agreement, FP32 reversibility"""
+    gen = gf.generators
+    L64 = make_lattice(gf, dims, "f64", seed_u=777, seed_p=778)
+    L64.stout_config(3, 0.12, "store")
+    dh_therm = []
+    for k in range(ntherm):
+        L64.upload_momenta(gen.gaussian_momenta(dims, 2000 + k))
+        dh_therm.append(L64.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True)[2])
+    U0 = L64.download_links()
+    P0 = gen.gaussian_momenta(dims, 9999)
+    L32 = gf.Lattice(dims, precision="f32")
+    L32.upload_links(U0)
+    L32.stout_config(3, 0.12, "store")
+    res = {}
+    for tag, L in (("fp64", L64), ("fp32", L32)):
+        times, dH = [], None
+        for rep in range(2):
+            L.upload_links(U0)
+            L.upload_momenta(P0)
+            L.sync()
+            t0 = time.perf_counter()
+            H = L.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True)
+            L.sync()
             times.append(time.perf_counter() - t0)
-            dH.append(H[2])
-    plaq = L.plaquette()
-    L.close()
-    return {"s_per_traj": float(np.mean(times)), "dH": dH, "plaquette": plaq}
+            dH = H[2]
+        res[tag] = {"s_per_traj": min(times), "dH": dH, "plaquette": L.plaquette()}
+    # FP32 reversibility: run back with tau -> -tau from the end point
+    L32.md_trajectory(COEFF, 16, -1.0, "2MN", fused=True, diagnostics=False)
+    Ub = L32.download_links()
+    rev = float(np.abs(Ub - U0).max())
+    L64.close()
+    L32.close()
+    return {
+        "lattice": "x".join(map(str, dims)), "levels": 3, "rho": 0.12, "integrator": "2MN", "steps": 16,
+        "tau": 1.0, "thermalization_dH": dh_therm,
+        "fp64_s_per_traj": res["fp64"]["s_per_traj"], "fp32_s_per_traj": res["fp32"]["s_per_traj"],
+        "speedup_fp32_over_fp64": res["fp64"]["s_per_traj"] / res["fp32"]["s_per_traj"],
+        "dH_fp64": res["fp64"]["dH"], "dH_fp32": res["fp32"]["dH"],
+        "abs_dH_diff": abs(res["fp64"]["dH"] - res["fp32"]["dH"]),
+        "plaquette_diff": abs(res["fp64"]["plaquette"] - res["fp32"]["plaquette"]),
+        "fp32_reversibility_max_abs_dU": rev,
+    }
 
 
 def main():
@@ -106,7 +188,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-stout", action="store_true", help="skip the stout-HMC s/traj lines")
-    ap.add_argument("--stout-traj", type=int, default=2)
+    ap.add_argument("--stout-therm", type=int, default=6, help="FP64 thermalization trajectories")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -140,6 +222,10 @@ def main():
     ms_force = time_kernel(L, lambda: L.force_kick(COEFF, 0.0), 10)
     ms_exp = time_kernel(L, lambda: L.exp_update(0.02), 10)
     ms_plaq = time_kernel(L, lambda: L.plaquette_sum(), 10)
+    # device-to-device copy of the link field (hipMemcpyAsync): the measured stream-copy
+    # rate, and the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters
+    ms_copy = time_kernel(L, L.save_links, 10)
+    copy_gbps = 2 * 72 * 8 * V / (ms_copy * 1e-3) / 1e9
     plaq = L.plaquette()
     L.close()
     kernels = {
@@ -152,22 +238,16 @@ def main():
         "plaquette": {"ms_incl_readback": ms_plaq, "bytes_per_site": 576,
                       "GBps": 576 * V / (ms_plaq * 1e-3) / 1e9},
     }
+    kernels["link_copy"] = {"ms": ms_copy, "bytes": 2 * 72 * 8 * V, "GBps": copy_gbps}
     log(f"[bench] kernels {json.dumps(kernels)}")
 
     stout = None
+    flow = None
     if not args.no_stout:
-        s64 = stout_hmc(gf, "f64", args.stout_traj)
-        log(f"[bench] stout fp64 {s64}")
-        s32 = stout_hmc(gf, "f32", args.stout_traj)
-        log(f"[bench] stout fp32 {s32}")
-        stout = {
-            "lattice": "32^4", "levels": 3, "rho": 0.12, "integrator": "2MN", "steps": 16, "tau": 1.0,
-            "fp64_s_per_traj": s64["s_per_traj"], "fp32_s_per_traj": s32["s_per_traj"],
-            "speedup_fp32_over_fp64": s64["s_per_traj"] / s32["s_per_traj"],
-            "dH_fp64": s64["dH"], "dH_fp32": s32["dH"],
-            "max_abs_dH_diff": float(np.max(np.abs(np.array(s64["dH"]) - np.array(s32["dH"])))),
-            "plaquette_diff": abs(s64["plaquette"] - s32["plaquette"]),
-        }
+        stout = stout_hmc_thermalized(gf, ntherm=args.stout_therm)
+        log(f"[bench] stout {json.dumps(stout)}")
+        flow = flow_compare(gf)
+        log(f"[bench] flow {json.dumps(flow)}")
 
     cpu = None
     if not args.no_cpu:
@@ -191,9 +271,12 @@ def main():
                    "lattice": "32x32x32x32", "sites": V},
         "site_updates_per_s": V / (ms_per_step * 1e-3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": recorded_traffic("void gf::k_force<double, 2>")},
+        "measured_copy_GBps": copy_gbps,
         "kernels": kernels,
         "stout_hmc": stout,
+        "wilson_flow": flow,
         "plaquette": plaq,
         "cpu_baseline": cpu,
     }

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <algorithm>
 #include <utility>
 
 #include "../../include/gfhip.h"
@@ -247,8 +248,9 @@ int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
         if (dims[m] <= 0) return GF_EINVAL;
         V *= dims[m];
     }
-    // 72 reals per site must stay addressable with 32-bit site indices
-    if (V * 72 > (1LL << 31) - 1) return GF_EINVAL;
+    // one direction of one field (9 complex planes of V sites) is one buffer descriptor whose
+    // byte extent must fit the descriptor's 32-bit record count
+    if (V * 9 * 2 * (precision == GF_F64 ? 8 : 4) > 0xFFFFFFFFLL) return GF_EINVAL;
     if (hipSetDevice(device) != hipSuccess) return GF_EDEVICE;
     gf_ctx* c = new gf_ctx();
     for (int m = 0; m < 4; m++) c->dims[m] = dims[m];
@@ -259,6 +261,26 @@ int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
         c->geo.n[m] = dims[m];
         c->geo.st[m] = st;
         st *= dims[m];
+    }
+    // Morton order of the 64-site groups over (xy-group, z, t) when every extent is a power of two
+    {
+        auto log2_exact = [](long long n) -> int {
+            if (n <= 0 || (n & (n - 1))) return -1;
+            int b = 0;
+            while ((1LL << b) < n) b++;
+            return b;
+        };
+        const long long nxy = (long long)dims[0] * dims[1];
+        const int bxy = nxy % 64 == 0 ? log2_exact(nxy / 64) : -1;
+        const int bz = log2_exact(dims[2]), bt = log2_exact(dims[3]);
+        if (bxy >= 0 && bz >= 0 && bt >= 0) {
+            c->geo.mbits[0] = bxy;
+            c->geo.mbits[1] = bz;
+            c->geo.mbits[2] = bt;
+        } else {
+            c->geo.mbits[0] = -1;
+            c->geo.mbits[1] = c->geo.mbits[2] = 0;
+        }
     }
     c->prec = precision;
     c->esz = precision == GF_F64 ? 8 : 4;
@@ -271,7 +293,8 @@ int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
     for (int i = 0; i < GF_NEVENTS; i++)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) return bail(GF_EDEVICE);
     if (alloc(c, &c->U, link_bytes(c)) || alloc(c, &c->Ualt, link_bytes(c)) || alloc(c, &c->P, mom_bytes(c)) ||
-        alloc(c, &c->F, mom_bytes(c)) || alloc(c, (void**)&c->partial, sizeof(double) * grid_for(c->V)) ||
+        alloc(c, &c->F, mom_bytes(c)) ||
+        alloc(c, (void**)&c->partial, sizeof(double) * std::max(grid_for(c->V), grid_sites_mu(c->V))) ||
         alloc(c, (void**)&c->dres, sizeof(double)))
         return bail(GF_ENOMEM);
     if (hipHostMalloc((void**)&c->hres, sizeof(double), hipHostMallocDefault) != hipSuccess) return bail(GF_ENOMEM);

@@ -11,6 +11,8 @@ namespace gf {
 constexpr int kBlock = 256;
 
 inline int grid_for(int V, int block = kBlock) { return (V + block - 1) / block; }
+// grid of the (site, direction) kernels: one 256-thread workgroup per 64 sites
+inline int grid_sites_mu(int V) { return (V + 63) / 64; }
 
 // force kernel modes
 enum ForceMode { FORCE_STORE = 0, FORCE_KICK = 1, FORCE_KICK_EXP = 2 };

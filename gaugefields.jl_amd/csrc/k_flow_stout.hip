@@ -13,30 +13,28 @@ namespace gf {
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restrict__ W, T* __restrict__ Wout,
                                                        T* __restrict__ A, T alpha, T beta) {
-    const int s = site_index();
+    const SiteMu sm = site_mu_index(g);
+    const int s = sm.s, mu = sm.mu;
     if (s >= g.V) return;
     int x[4];
     decode(g, s, x);
-    const size_t V = g.V;
-#pragma unroll 1
-    for (int mu = 0; mu < 4; mu++) {
-        const M3<T> sig = staple_sigma(g, W, V, s, x, mu);
-        const M3<T> w = ldl(W, V, mu, s);
-        T c[8];
-        ta_coeffs(mul(w, sig), c);
-        if (beta != T(0)) {
-            T a[8];
-            ldp(A, V, mu, s, a);
+    const unsigned V = g.V;
+    const M3<T> sig = staple_sigma(g, W, V, s, x, mu);
+    const M3<T> w = ldl(W, V, mu, s);
+    T c[8];
+    ta_coeffs(mul(w, sig), c);
+    if (beta != T(0)) {
+        T a[8];
+        ldp(A, V, mu, s, a);
 #pragma unroll
-            for (int k = 0; k < 8; k++) c[k] = fma(alpha, c[k], beta * a[k]);
-        } else {
+        for (int k = 0; k < 8; k++) c[k] = fma(alpha, c[k], beta * a[k]);
+    } else {
 #pragma unroll
-            for (int k = 0; k < 8; k++) c[k] = alpha * c[k];
-        }
-        stp(A, V, mu, s, c);
-        const M3<T> E = expi_herm(herm_from_coeffs(c, T(1)));
-        stl(Wout, V, mu, s, mul(E, w));
+        for (int k = 0; k < 8; k++) c[k] = alpha * c[k];
     }
+    stp(A, V, mu, s, c);
+    const M3<T> E = expi_herm(herm_from_coeffs(c, T(1)));
+    stl(Wout, V, mu, s, mul(E, w));
 }
 
 // ---------------------------------------------------------------------------
@@ -47,19 +45,17 @@ __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restric
 template <typename T, bool STORE>
 __global__ __launch_bounds__(kBlock) void k_stout_forward(Geo g, const T* __restrict__ Uin, T* __restrict__ Uout,
                                                           T* __restrict__ Cst, T rho) {
-    const int s = site_index();
+    const SiteMu sm = site_mu_index(g);
+    const int s = sm.s, mu = sm.mu;
     if (s >= g.V) return;
     int x[4];
     decode(g, s, x);
-    const size_t V = g.V;
-#pragma unroll 1
-    for (int mu = 0; mu < 4; mu++) {
-        const M3<T> C = scal(rho, dag(staple_sigma(g, Uin, V, s, x, mu)));
-        const M3<T> u = ldl(Uin, V, mu, s);
-        const M3<T> E = exp_antiherm(ta_mat(mul_nd(C, u)));
-        stl(Uout, V, mu, s, mul(E, u));
-        if (STORE) stl(Cst, V, mu, s, C);
-    }
+    const unsigned V = g.V;
+    const M3<T> C = scal(rho, dag(staple_sigma(g, Uin, V, s, x, mu)));
+    const M3<T> u = ldl(Uin, V, mu, s);
+    const M3<T> E = exp_antiherm(ta_mat(mul_nd(C, u)));
+    stl(Uout, V, mu, s, mul(E, u));
+    if (STORE) stl(Cst, V, mu, s, C);
 }
 
 // ---------------------------------------------------------------------------
@@ -72,23 +68,26 @@ template <typename T, bool STORED>
 __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __restrict__ Uin,
                                                              const T* __restrict__ Cst, const T* __restrict__ Dout,
                                                              T* __restrict__ Din, T* __restrict__ G, T rho) {
-    const int s = site_index();
+    const SiteMu sm = site_mu_index(g);
+    const int s = sm.s, mu = sm.mu;
     if (s >= g.V) return;
-    int x[4];
-    decode(g, s, x);
-    const size_t V = g.V;
-#pragma unroll 1
-    for (int mu = 0; mu < 4; mu++) {
-        const M3<T> C = STORED ? ldl(Cst, V, mu, s) : scal(rho, dag(staple_sigma(g, Uin, V, s, x, mu)));
-        const M3<T> u = ldl(Uin, V, mu, s);
-        const M3<T> Q = ta_mat(mul_nd(C, u));
-        const M3<T> Do = ldl(Dout, V, mu, s);
-        M3<T> D = mul(Do, exp_antiherm(Q));
-        const M3<T> Lam = ta_mat(cdexpqdq(mul(u, Do), Q));
-        stl(G, V, mu, s, mul_dn(u, Lam));
-        acc_dag(D, mul(Lam, C));
-        stl(Din, V, mu, s, D);
+    const unsigned V = g.V;
+    M3<T> C;
+    if (STORED) {
+        C = ldl(Cst, V, mu, s);
+    } else {
+        int x[4];
+        decode(g, s, x);
+        C = scal(rho, dag(staple_sigma(g, Uin, V, s, x, mu)));
     }
+    const M3<T> u = ldl(Uin, V, mu, s);
+    const M3<T> Q = ta_mat(mul_nd(C, u));
+    const M3<T> Do = ldl(Dout, V, mu, s);
+    M3<T> D = mul(Do, exp_antiherm(Q));
+    const M3<T> Lam = ta_mat(cdexpqdq(mul(u, Do), Q));
+    stl(G, V, mu, s, mul_dn(u, Lam));
+    acc_dag(D, mul(Lam, C));
+    stl(Din, V, mu, s, D);
 }
 
 // ---------------------------------------------------------------------------
@@ -102,13 +101,13 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* __restrict__ U,
                                                                const T* __restrict__ G, T* __restrict__ Din, T rho) {
-    const int y = site_index();
+    const SiteMu sm = site_mu_index(g);
+    const int y = sm.s, a = sm.mu;
     if (y >= g.V) return;
     int x[4];
     decode(g, y, x);
-    const size_t V = g.V;
-#pragma unroll 1
-    for (int a = 0; a < 4; a++) {
+    const unsigned V = g.V;
+    {
         M3<T> acc = zero3<T>();
         const int ypa = y + dlt(g, x, a, 1);
 #pragma unroll
@@ -148,27 +147,27 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* _
 // ---------------------------------------------------------------------------
 template <typename T>
 hipError_t launch_flow_stage(const Geo& g, const T* W, T* Wout, T* A, double alpha, double beta, hipStream_t st) {
-    k_flow_stage<T><<<grid_for(g.V), kBlock, 0, st>>>(g, W, Wout, A, (T)alpha, (T)beta);
+    k_flow_stage<T><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, W, Wout, A, (T)alpha, (T)beta);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_stout_forward(const Geo& g, const T* Uin, T* Uout, T* Cst, double rho, hipStream_t st) {
     if (Cst)
-        k_stout_forward<T, true><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, Uout, Cst, (T)rho);
+        k_stout_forward<T, true><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Uout, Cst, (T)rho);
     else
-        k_stout_forward<T, false><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, Uout, nullptr, (T)rho);
+        k_stout_forward<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Uout, nullptr, (T)rho);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* G, double rho,
                                  hipStream_t st) {
     if (Cst)
-        k_stout_back_local<T, true><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, Cst, Dout, Din, G, (T)rho);
+        k_stout_back_local<T, true><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Cst, Dout, Din, G, (T)rho);
     else
-        k_stout_back_local<T, false><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, G, (T)rho);
+        k_stout_back_local<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, G, (T)rho);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_stout_back_stencil<T><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, G, Din, (T)rho);
+    k_stout_back_stencil<T><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, G, Din, (T)rho);
     return hipGetLastError();
 }
 

@@ -1,9 +1,9 @@
 // k_gauge.hip -- plaquette/action sums, staple derivative, fused force + momentum kick
 // (+ exp link update), exp link update, layout conversion.
 //
-// One thread per lattice site; each thread handles all four link directions of its site.
-// Neighbour links are re-read through L1/L2 (the XCD-aware block order keeps a site's
-// halo in the same XCD's L2); HBM sees each input plane once per launch.
+// Stencil kernels run one thread per (site, link direction) (site_mu_index): the four
+// directions of a group of 64 sites are the four wavefronts of one workgroup, and groups are
+// visited in 4D Morton order per XCD, so neighbour links are re-read from L1/L2 instead of HBM.
 #include "internal.hpp"
 
 namespace gf {
@@ -14,24 +14,22 @@ namespace gf {
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_plaquette(Geo g, const T* __restrict__ U, double* __restrict__ partial) {
-    const int s = site_index();
+    const SiteMu sm = site_mu_index(g);
+    const int s = sm.s;
     double v = 0.0;
     if (s < g.V) {
         int x[4];
         decode(g, s, x);
-        const size_t V = g.V;
-#pragma unroll
-        for (int mu = 0; mu < 3; mu++) {
-            const int spm = s + dlt(g, x, mu, 1);
-            const M3<T> umu = ldl(U, V, mu, s);
-#pragma unroll
-            for (int nu = mu + 1; nu < 4; nu++) {
-                int s0 = s, s1 = spm, spn = s + dlt(g, x, nu, 1);
-                asm volatile("" : "+v"(s0), "+v"(s1), "+v"(spn));  // no cross-plane load merging
-                const M3<T> a = mul(umu, ldl(U, V, nu, s1));                // U_mu(x) U_nu(x+mu)
-                const M3<T> b = mul(ldl(U, V, nu, s0), ldl(U, V, mu, spn));  // U_nu(x) U_mu(x+nu)
-                v += (double)retr_nd(a, b);
-            }
+        const unsigned V = g.V;
+        // the six planes (0,1) (0,2) (0,3) (1,2) (1,3) (2,3): wavefront w takes planes w and w+4
+#pragma unroll 1
+        for (int pl = sm.mu; pl < 6; pl += 4) {
+            const int mu = pl < 3 ? 0 : (pl < 5 ? 1 : 2);
+            const int nu = pl < 3 ? pl + 1 : (pl < 5 ? pl - 1 : 3);
+            const int spm = s + dlt(g, x, mu, 1), spn = s + dlt(g, x, nu, 1);
+            const M3<T> a = mul(ldl(U, V, mu, s), ldl(U, V, nu, spm));  // U_mu(x) U_nu(x+mu)
+            const M3<T> b = mul(ldl(U, V, nu, s), ldl(U, V, mu, spn));  // U_nu(x) U_mu(x+nu)
+            v += (double)retr_nd(a, b);
         }
     }
     block_sum_store<kBlock>(v, partial);
@@ -40,15 +38,13 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(Geo g, const T* __restrict
 // sum of p_a^2 over all links (p*p, TA_gaugefields_4D_serial.jl:106-129)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_momenta_dot(Geo g, const T* __restrict__ P, double* __restrict__ partial) {
-    const int s = site_index();
+    const SiteMu sm = site_mu_index(g);
     double v = 0.0;
-    if (s < g.V) {
-        const size_t V = g.V;
+    if (sm.s < g.V) {
+        T c[8];
+        ldp(P, (unsigned)g.V, sm.mu, (unsigned)sm.s, c);
 #pragma unroll
-        for (int k = 0; k < 32; k++) {
-            const double p = (double)P[k * V + s];
-            v = fma(p, p, v);
-        }
+        for (int a = 0; a < 8; a++) v = fma((double)c[a], (double)c[a], v);
     }
     block_sum_store<kBlock>(v, partial);
 }
@@ -68,16 +64,14 @@ __global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict_
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// K2: dSdU_mu = coeff * Sigma_mu for all four directions
+// K2: dSdU_mu = coeff * Sigma_mu
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U, T coeff, T* __restrict__ D) {
-    const int s = site_index();
-    if (s >= g.V) return;
+    const SiteMu sm = site_mu_index(g);
+    if (sm.s >= g.V) return;
     int x[4];
-    decode(g, s, x);
-    const size_t V = g.V;
-#pragma unroll 1
-    for (int mu = 0; mu < 4; mu++) stl(D, V, mu, s, scal(coeff, staple_sigma(g, U, V, s, x, mu)));
+    decode(g, sm.s, x);
+    stl(D, (unsigned)g.V, sm.mu, (unsigned)sm.s, scal(coeff, staple_sigma(g, U, (unsigned)g.V, sm.s, x, sm.mu)));
 }
 
 // K3 (+K4): F_mu = -(coeff/3) TA(U_mu Sigma_mu)   (md_force!, molecular_dynamics.jl:251-267)
@@ -87,31 +81,29 @@ __global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U,
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U, T* __restrict__ FP,
                                                   T* __restrict__ Unew, T fac, T eps_u) {
-    const int s = site_index();
+    const SiteMu sm = site_mu_index(g);
+    const int s = sm.s, mu = sm.mu;
     if (s >= g.V) return;
     int x[4];
     decode(g, s, x);
-    const size_t V = g.V;
-#pragma unroll 1
-    for (int mu = 0; mu < 4; mu++) {
-        const M3<T> sig = staple_sigma(g, U, V, s, x, mu);
-        const M3<T> umu = ldl(U, V, mu, s);
-        T c[8];
-        ta_coeffs(mul(umu, sig), c);
-        if (MODE == FORCE_STORE) {
+    const unsigned V = g.V;
+    const M3<T> sig = staple_sigma(g, U, V, s, x, mu);
+    const M3<T> umu = ldl(U, V, mu, s);
+    T c[8];
+    ta_coeffs(mul(umu, sig), c);
+    if (MODE == FORCE_STORE) {
 #pragma unroll
-            for (int a = 0; a < 8; a++) c[a] *= fac;
-            stp(FP, V, mu, s, c);
-        } else {
-            T p[8];
-            ldp(FP, V, mu, s, p);
+        for (int a = 0; a < 8; a++) c[a] *= fac;
+        stp(FP, V, mu, s, c);
+    } else {
+        T p[8];
+        ldp(FP, V, mu, s, p);
 #pragma unroll
-            for (int a = 0; a < 8; a++) p[a] = fma(fac, c[a], p[a]);
-            stp(FP, V, mu, s, p);
-            if (MODE == FORCE_KICK_EXP) {
-                const M3<T> E = expi_herm(herm_from_coeffs(p, eps_u));
-                stl(Unew, V, mu, s, mul(E, umu));
-            }
+        for (int a = 0; a < 8; a++) p[a] = fma(fac, c[a], p[a]);
+        stp(FP, V, mu, s, p);
+        if (MODE == FORCE_KICK_EXP) {
+            const M3<T> E = expi_herm(herm_from_coeffs(p, eps_u));
+            stl(Unew, V, mu, s, mul(E, umu));
         }
     }
 }
@@ -120,16 +112,13 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_exp_update(Geo g, const T* __restrict__ Uin, T* __restrict__ Uout,
                                                        const T* __restrict__ P, T eps) {
-    const int s = site_index();
-    if (s >= g.V) return;
-    const size_t V = g.V;
-#pragma unroll 1
-    for (int mu = 0; mu < 4; mu++) {
-        T p[8];
-        ldp(P, V, mu, s, p);
-        const M3<T> E = expi_herm(herm_from_coeffs(p, eps));
-        stl(Uout, V, mu, s, mul(E, ldl(Uin, V, mu, s)));
-    }
+    const SiteMu sm = site_mu_index(g);
+    if (sm.s >= g.V) return;
+    const unsigned V = g.V;
+    T p[8];
+    ldp(P, V, sm.mu, sm.s, p);
+    const M3<T> E = expi_herm(herm_from_coeffs(p, eps));
+    stl(Uout, V, sm.mu, sm.s, mul(E, ldl(Uin, V, sm.mu, sm.s)));
 }
 
 // F_mu = -(1/3) TA(U_mu D_mu) from a matrix derivative D (stout force,
@@ -137,26 +126,24 @@ __global__ __launch_bounds__(kBlock) void k_exp_update(Geo g, const T* __restric
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_force_from_deriv(Geo g, const T* __restrict__ U, const T* __restrict__ D,
                                                              T* __restrict__ FP, T eps) {
-    const int s = site_index();
+    const SiteMu sm = site_mu_index(g);
+    const int s = sm.s, mu = sm.mu;
     if (s >= g.V) return;
-    const size_t V = g.V;
+    const unsigned V = g.V;
     const T fac = T(-1.0 / 3.0);
-#pragma unroll 1
-    for (int mu = 0; mu < 4; mu++) {
-        T c[8];
-        ta_coeffs(mul(ldl(U, V, mu, s), ldl(D, V, mu, s)), c);
-        if (MODE == FORCE_STORE) {
+    T c[8];
+    ta_coeffs(mul(ldl(U, V, mu, s), ldl(D, V, mu, s)), c);
+    if (MODE == FORCE_STORE) {
 #pragma unroll
-            for (int a = 0; a < 8; a++) c[a] *= fac;
-            stp(FP, V, mu, s, c);
-        } else {
-            T p[8];
-            ldp(FP, V, mu, s, p);
-            const T f = eps * fac;
+        for (int a = 0; a < 8; a++) c[a] *= fac;
+        stp(FP, V, mu, s, c);
+    } else {
+        T p[8];
+        ldp(FP, V, mu, s, p);
+        const T f = eps * fac;
 #pragma unroll
-            for (int a = 0; a < 8; a++) p[a] = fma(f, c[a], p[a]);
-            stp(FP, V, mu, s, p);
-        }
+        for (int a = 0; a < 8; a++) p[a] = fma(f, c[a], p[a]);
+        stp(FP, V, mu, s, p);
     }
 }
 
@@ -168,33 +155,34 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_links_from_ref(const double* __restrict__ ref, T* __restrict__ soa, int V) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= V) return;
-    const size_t Vs = V;
     for (int mu = 0; mu < 4; mu++) {
-        const double* r = ref + ((size_t)mu * Vs + s) * 18;
+        const double* r = ref + ((size_t)mu * V + s) * 18;
+        M3<T> m;
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                const int k = 3 * i + j, c = i + 3 * j;
-                soa[((size_t)mu * 18 + 2 * k) * Vs + s] = (T)r[2 * c];
-                soa[((size_t)mu * 18 + 2 * k + 1) * Vs + s] = (T)r[2 * c + 1];
+                const int c = i + 3 * j;  // column-major complex index of the reference
+                m.m[3 * i + j].r = (T)r[2 * c];
+                m.m[3 * i + j].i = (T)r[2 * c + 1];
             }
+        stl(soa, (unsigned)V, mu, (unsigned)s, m);
     }
 }
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_links_to_ref(const T* __restrict__ soa, double* __restrict__ ref, int V) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= V) return;
-    const size_t Vs = V;
     for (int mu = 0; mu < 4; mu++) {
-        double* r = ref + ((size_t)mu * Vs + s) * 18;
+        double* r = ref + ((size_t)mu * V + s) * 18;
+        const M3<T> m = ldl(soa, (unsigned)V, mu, (unsigned)s);
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                const int k = 3 * i + j, c = i + 3 * j;
-                r[2 * c] = (double)soa[((size_t)mu * 18 + 2 * k) * Vs + s];
-                r[2 * c + 1] = (double)soa[((size_t)mu * 18 + 2 * k + 1) * Vs + s];
+                const int c = i + 3 * j;
+                r[2 * c] = (double)m.m[3 * i + j].r;
+                r[2 * c + 1] = (double)m.m[3 * i + j].i;
             }
     }
 }
@@ -202,28 +190,29 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_momenta_from_ref(const double* __restrict__ ref, T* __restrict__ soa, int V) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= V) return;
-    const size_t Vs = V;
-    for (int mu = 0; mu < 4; mu++)
+    for (int mu = 0; mu < 4; mu++) {
+        T c[8];
 #pragma unroll
-        for (int a = 0; a < 8; a++) soa[((size_t)mu * 8 + a) * Vs + s] = (T)ref[((size_t)mu * Vs + s) * 8 + a];
+        for (int a = 0; a < 8; a++) c[a] = (T)ref[((size_t)mu * V + s) * 8 + a];
+        stp(soa, (unsigned)V, mu, (unsigned)s, c);
+    }
 }
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_momenta_to_ref(const T* __restrict__ soa, double* __restrict__ ref, int V) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= V) return;
-    const size_t Vs = V;
-    for (int mu = 0; mu < 4; mu++)
+    for (int mu = 0; mu < 4; mu++) {
+        T c[8];
+        ldp(soa, (unsigned)V, mu, (unsigned)s, c);
 #pragma unroll
-        for (int a = 0; a < 8; a++) ref[((size_t)mu * Vs + s) * 8 + a] = (double)soa[((size_t)mu * 8 + a) * Vs + s];
+        for (int a = 0; a < 8; a++) ref[((size_t)mu * V + s) * 8 + a] = (double)c[a];
+    }
 }
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_set_identity(T* __restrict__ U, int V) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= V) return;
-    const size_t Vs = V;
-    for (int mu = 0; mu < 4; mu++)
-#pragma unroll
-        for (int r = 0; r < 18; r++) U[((size_t)mu * 18 + r) * Vs + s] = (r == 0 || r == 8 || r == 16) ? T(1) : T(0);
+    for (int mu = 0; mu < 4; mu++) stl(U, (unsigned)V, mu, (unsigned)s, ident3<T>());
 }
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill(T* __restrict__ p, size_t n, T v) {
@@ -241,27 +230,27 @@ __global__ __launch_bounds__(kBlock) void k_axpby(T* __restrict__ y, const T* __
 // ---------------------------------------------------------------------------
 template <typename T>
 hipError_t launch_plaquette(const Geo& g, const T* U, double* partial, double* out, hipStream_t st) {
-    const int nb = grid_for(g.V);
+    const int nb = grid_sites_mu(g.V);
     k_plaquette<T><<<nb, kBlock, 0, st>>>(g, U, partial);
     k_sum_partials<<<1, 1024, 0, st>>>(partial, nb, out);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_momenta_dot(const Geo& g, const T* P, double* partial, double* out, hipStream_t st) {
-    const int nb = grid_for(g.V);
+    const int nb = grid_sites_mu(g.V);
     k_momenta_dot<T><<<nb, kBlock, 0, st>>>(g, P, partial);
     k_sum_partials<<<1, 1024, 0, st>>>(partial, nb, out);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_dsdu(const Geo& g, const T* U, double coeff, T* D, hipStream_t st) {
-    k_dsdu<T><<<grid_for(g.V), kBlock, 0, st>>>(g, U, (T)coeff, D);
+    k_dsdu<T><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, (T)coeff, D);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_force(int mode, const Geo& g, const T* U, T* FP, T* Unew, double coeff, double eps_p,
                         double eps_u, hipStream_t st) {
-    const int nb = grid_for(g.V);
+    const int nb = grid_sites_mu(g.V);
     const double fac = -coeff / 3.0;
     switch (mode) {
         case FORCE_STORE:
@@ -280,16 +269,16 @@ hipError_t launch_force(int mode, const Geo& g, const T* U, T* FP, T* Unew, doub
 }
 template <typename T>
 hipError_t launch_exp_update(const Geo& g, const T* Uin, T* Uout, const T* P, double eps, hipStream_t st) {
-    k_exp_update<T><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, Uout, P, (T)eps);
+    k_exp_update<T><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Uout, P, (T)eps);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_force_from_deriv(int mode, const Geo& g, const T* U, const T* D, T* FP, double eps,
                                    hipStream_t st) {
     if (mode == FORCE_STORE)
-        k_force_from_deriv<T, FORCE_STORE><<<grid_for(g.V), kBlock, 0, st>>>(g, U, D, FP, (T)eps);
+        k_force_from_deriv<T, FORCE_STORE><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, D, FP, (T)eps);
     else if (mode == FORCE_KICK)
-        k_force_from_deriv<T, FORCE_KICK><<<grid_for(g.V), kBlock, 0, st>>>(g, U, D, FP, (T)eps);
+        k_force_from_deriv<T, FORCE_KICK><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, D, FP, (T)eps);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -1,10 +1,12 @@
 // lattice.hpp -- site geometry and structure-of-arrays field access for the gfx950 kernels.
 //
-// HBM layout (one allocation per field, T = double or float):
-//   link field     : 4 directions x 18 real planes x V sites; plane (mu*18 + r), r = 2*(3*i + j) + {0:re,1:im}
-//   momentum field : 4 directions x  8 real planes x V sites; plane (mu*8 + a), coefficient of i lambda_a/2
+// HBM layout (one allocation per field, T = double or float, T2 = a (T, T) pair):
+//   link field     : 4 directions x 9 complex planes x V sites of T2; plane (mu*9 + 3*i + j)
+//                    holds element (i, j) of U_mu as (re, im)
+//   momentum field : 4 directions x 4 pair planes x V sites of T2; plane (mu*4 + j) holds the
+//                    coefficients (2j, 2j+1) of i lambda_a/2
 // Consecutive sites x, x+1, ... are consecutive in every plane, so one wavefront of 64
-// sites reads each plane with fully coalesced 256 B (FP32) / 512 B (FP64) transactions.
+// sites reads each plane with one fully coalesced 1 KiB (FP64) / 512 B (FP32) access.
 // Site order: s = x + NX*(y + NY*(z + NZ*t)), periodic in all four directions.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -17,6 +19,9 @@ struct Geo {
     int n[4];   // NX, NY, NZ, NT
     int st[4];  // site strides 1, NX, NX*NY, NX*NY*NZ
     int V;
+    // Morton bit widths of the 64-site group coordinates (xy-group, z, t); mbits[0] < 0
+    // selects plain linear group order (extents that are not powers of two)
+    int mbits[3];
 };
 
 __device__ __forceinline__ void decode(const Geo& g, int s, int x[4]) {
@@ -36,50 +41,144 @@ __device__ __forceinline__ int dlt(const Geo& g, const int x[4], int mu, int d) 
     return xm >= g.n[mu] ? -wrap : (xm < 0 ? wrap : d * g.st[mu]);
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).  Remap so
-// that XCD k walks the contiguous k-th eighth of the lattice: the halo rows a site needs
-// (x +- y, z, t neighbours) were then fetched recently by the same XCD and sit in its L2.
-__device__ __forceinline__ int site_index() {
+// Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).  xcd_block()
+// renumbers them so that XCD k runs the contiguous k-th eighth of the block sequence.
+__device__ __forceinline__ unsigned xcd_block() {
     unsigned b = blockIdx.x;
     const unsigned nb = gridDim.x;
     if ((nb & 7u) == 0u) b = (b & 7u) * (nb >> 3) + (b >> 3);
-    return (int)(b * blockDim.x + threadIdx.x);
+    return b;
 }
 
+// one thread per site (legacy site-parallel kernels: reductions, layout conversion)
+__device__ __forceinline__ int site_index() { return (int)(xcd_block() * blockDim.x + threadIdx.x); }
+
+// Thread <-> (site, direction) map of the stencil kernels.  A workgroup of 256 threads is
+// 4 wavefronts over the same group of 64 consecutive sites; wavefront w handles link
+// direction mu = w.  The four directions of a site therefore run on one CU at the same
+// time and share their neighbour links through L1/L2.  Groups are visited in 4D Morton
+// order over (xy-group, z, t) so that the groups in flight on one XCD form a compact 4D
+// region whose halo stays in that XCD's L2.
+__device__ __forceinline__ int group_of_block(const Geo& g) {
+    const unsigned b = xcd_block();
+    if (g.mbits[0] < 0) return (int)b;
+    unsigned v0 = 0, v1 = 0, v2 = 0, bit = 0;
+    const int top = max(g.mbits[0], max(g.mbits[1], g.mbits[2]));
+    for (int lv = 0; lv < top; lv++) {
+        if (lv < g.mbits[0]) v0 |= ((b >> bit++) & 1u) << lv;
+        if (lv < g.mbits[1]) v1 |= ((b >> bit++) & 1u) << lv;
+        if (lv < g.mbits[2]) v2 |= ((b >> bit++) & 1u) << lv;
+    }
+    return (int)(v0 + ((v1 + (v2 << g.mbits[1])) << g.mbits[0]));
+}
+
+struct SiteMu {
+    int s;   // site
+    int mu;  // link direction (wave-uniform)
+};
+__device__ __forceinline__ SiteMu site_mu_index(const Geo& g) {
+    SiteMu r;
+    r.s = group_of_block(g) * 64 + (int)(threadIdx.x & 63u);
+    r.mu = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    return r;
+}
+
+// Field planes hold complex pairs (re, im) of type T2: one 16 B (FP64) or 8 B (FP32) access
+// per lane per plane.  Accesses are raw buffer loads/stores: the buffer descriptor (one
+// direction of one field) and the plane offset are wave-uniform scalars, the per-lane part is
+// the 32-bit byte offset of the site, so no vector address arithmetic is spent per access.
+// Offsets past the descriptor's extent read as zero and drop writes.
 template <typename T>
-__device__ __forceinline__ M3<T> ldl(const T* __restrict__ p, size_t V, int mu, int s) {
-    const T* q = p + (size_t)mu * 18 * V + s;
+struct pair_of;
+template <>
+struct pair_of<double> {
+    using type = double2;
+};
+template <>
+struct pair_of<float> {
+    using type = float2;
+};
+template <typename T>
+using T2 = typename pair_of<T>::type;
+
+#define GF_RSRC_WORD3 0x00020000  // raw buffer, 32-bit data format (gfx9 family)
+
+// descriptor over `planes` consecutive planes of V pairs starting at plane `first`
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t planes_rsrc(const T* p, unsigned V, unsigned first,
+                                                             unsigned planes) {
+    const char* base = reinterpret_cast<const char*>(p) + (size_t)first * V * sizeof(T2<T>);
+    return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)(planes * V * (unsigned)sizeof(T2<T>)),
+                                             GF_RSRC_WORD3);
+}
+
+__device__ __forceinline__ double2 buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, const double*) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ float2 buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, const float*) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void buf_st(double2 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(HIP_vector_type<unsigned, 4>::Native_vec_, v), r,
+                                           (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void buf_st(float2 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(HIP_vector_type<unsigned, 2>::Native_vec_, v), r,
+                                          (int)voff, (int)soff, 0);
+}
+
+// link field: plane (mu*9 + k) holds complex element k = 3*i + j of U_mu for all sites
+template <typename T>
+__device__ __forceinline__ M3<T> ldl(const T* __restrict__ p, unsigned V, int mu, unsigned s) {
+    const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 9u * mu, 9u);
+    const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
     M3<T> m;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        m.m[k].r = q[(2 * k) * V];
-        m.m[k].i = q[(2 * k + 1) * V];
+        const T2<T> v = buf_ld(r, off, k * ps, (const T*)nullptr);
+        m.m[k].r = v.x;
+        m.m[k].i = v.y;
     }
     return m;
 }
 
 template <typename T>
-__device__ __forceinline__ void stl(T* __restrict__ p, size_t V, int mu, int s, const M3<T>& m) {
-    T* q = p + (size_t)mu * 18 * V + s;
+__device__ __forceinline__ void stl(T* __restrict__ p, unsigned V, int mu, unsigned s, const M3<T>& m) {
+    const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 9u * mu, 9u);
+    const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        q[(2 * k) * V] = m.m[k].r;
-        q[(2 * k + 1) * V] = m.m[k].i;
+        T2<T> v;
+        v.x = m.m[k].r;
+        v.y = m.m[k].i;
+        buf_st(v, r, off, k * ps);
+    }
+}
+
+// momentum field: plane (mu*4 + j) holds coefficients (2j, 2j+1) of P_mu for all sites
+template <typename T>
+__device__ __forceinline__ void ldp(const T* __restrict__ p, unsigned V, int mu, unsigned s, T c[8]) {
+    const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 4u * mu, 4u);
+    const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const T2<T> v = buf_ld(r, off, j * ps, (const T*)nullptr);
+        c[2 * j] = v.x;
+        c[2 * j + 1] = v.y;
     }
 }
 
 template <typename T>
-__device__ __forceinline__ void ldp(const T* __restrict__ p, size_t V, int mu, int s, T c[8]) {
-    const T* q = p + (size_t)mu * 8 * V + s;
+__device__ __forceinline__ void stp(T* __restrict__ p, unsigned V, int mu, unsigned s, const T c[8]) {
+    const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 4u * mu, 4u);
+    const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
 #pragma unroll
-    for (int a = 0; a < 8; a++) c[a] = q[a * V];
-}
-
-template <typename T>
-__device__ __forceinline__ void stp(T* __restrict__ p, size_t V, int mu, int s, const T c[8]) {
-    T* q = p + (size_t)mu * 8 * V + s;
-#pragma unroll
-    for (int a = 0; a < 8; a++) q[a * V] = c[a];
+    for (int j = 0; j < 4; j++) {
+        T2<T> v;
+        v.x = c[2 * j];
+        v.y = c[2 * j + 1];
+        buf_st(v, r, off, j * ps);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -91,7 +190,7 @@ __device__ __forceinline__ void stp(T* __restrict__ p, size_t V, int mu, int s, 
 // of stout smearing (stout_fast.jl:603-624).
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ M3<T> staple_sigma(const Geo& g, const T* __restrict__ U, size_t V, int s,
+__device__ __forceinline__ M3<T> staple_sigma(const Geo& g, const T* __restrict__ U, unsigned V, int s,
                                               const int x[4], int mu) {
     M3<T> sig = zero3<T>();
     const int spm = s + dlt(g, x, mu, 1);

@@ -91,7 +91,7 @@ def test_plaquette_16_gauge_invariant(Lat, oracle):
 # ------------------------------------------------------------------ derivative, force, exp
 
 
-@pytest.mark.parametrize("dims", [D4, (4, 2, 3, 2)])
+@pytest.mark.parametrize("dims", [D4, (4, 2, 3, 2), (16, 8, 4, 8)])
 def test_dsdu_parity(Lat, oracle, dims):
     L = Lat(dims)
     U = gen.hot_start_links(dims, 11)
@@ -99,7 +99,7 @@ def test_dsdu_parity(Lat, oracle, dims):
     assert maxdiff(L.dsdu(COEFF), oracle.dsdu(dims, U, COEFF)) < 1e-13 * COEFF * 6
 
 
-@pytest.mark.parametrize("dims", [D4, (2, 4, 2, 6)])
+@pytest.mark.parametrize("dims", [D4, (2, 4, 2, 6), (8, 8, 8, 8), (16, 8, 4, 8)])
 def test_force_parity(Lat, oracle, dims):
     L = Lat(dims)
     U = alg_links(oracle, dims, 0.3, 12)
@@ -258,3 +258,36 @@ def test_fp32_bulk_against_fp64_oracle(Lat, oracle):
     H = L.md_trajectory(COEFF, 10, 1.0, "QPQ", fused=True)
     Href = oracle.md_trajectory(D4, U.copy(), P.copy(), COEFF, 10, 1.0, "QPQ")
     assert abs(H[2] - Href[2]) < 1e-3
+
+
+# ------------------------------------------------------------------ Morton-ordered traversal
+# Extents whose xy-plane holds a multiple of 64 sites and whose extents are powers of two run
+# the stencil kernels in 4D Morton group order; these cases pin that path.
+
+
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (16, 8, 4, 8), (32, 4, 2, 4)])
+def test_morton_path_md_and_flow(Lat, oracle, dims):
+    L = Lat(dims)
+    U = alg_links(oracle, dims, 0.3, 61)
+    P = gen.gaussian_momenta(dims, 62)
+    L.upload_links(U)
+    L.upload_momenta(P)
+    H = L.md_trajectory(COEFF, 3, 0.3, "2MN", fused=True)
+    Href = oracle.md_trajectory(dims, U, P, COEFF, 3, 0.3, "2MN")
+    assert abs(H[2] - Href[2]) < 1e-12 * abs(Href[0])
+    assert maxdiff(L.download_links(), U) < 1e-12
+    E = L.flow(0.02, 4, 2)
+    Eref = oracle.flow(dims, U, 0.02, 4, 2)
+    assert np.all(np.abs(E - Eref) < 1e-12 * np.abs(Eref))
+    assert rel(L.plaquette_sum(), oracle.plaquette_sum(dims, U)) < 1e-12
+
+
+def test_morton_path_stout(Lat, oracle):
+    dims = (8, 8, 8, 8)
+    L = Lat(dims)
+    U = alg_links(oracle, dims, 0.3, 63)
+    L.upload_links(U)
+    L.stout_config(3, 0.1, "recompute")
+    L.force(COEFF)
+    F = oracle.stout_force(dims, U, 3, 0.1, COEFF)
+    assert maxdiff(L.download_force(), F) < 1e-12 * np.abs(F).max()
