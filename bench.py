@@ -82,7 +82,9 @@ def time_kernel(L, fn, reps):
     return L.event_elapsed_ms(2, 3) / reps
 
 
-def cpu_baseline(dims, updates):
+def cpu_baseline(dims, min_seconds=10.0, max_updates=200):
+    """The C oracle's MD update (force + kick + exp, FP64, OpenMP over the host's cores),
+    repeated until at least min_seconds of CPU work have run."""
     import oracle.gforacle as orc
 
     gf = __graft_entry__.load_package()
@@ -93,10 +95,12 @@ def cpu_baseline(dims, updates):
     V = int(np.prod(dims))
     F = np.zeros((4, V, 8))
     t0 = time.perf_counter()
-    for _ in range(updates):
+    updates = 0
+    while updates < max_updates and (updates < 2 or time.perf_counter() - t0 < min_seconds):
         F = orc.force(dims, U, COEFF)
         orc.kick(dims, P, F, 0.02)
         orc.exp_update(dims, U, P, 0.02)
+        updates += 1
     dt = time.perf_counter() - t0
     return {
         "value": 4 * V * updates / dt,
@@ -189,6 +193,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-stout", action="store_true", help="skip the stout-HMC s/traj lines")
     ap.add_argument("--stout-therm", type=int, default=6, help="FP64 thermalization trajectories")
+    ap.add_argument("--stout-only", action="store_true", help="run only the stout-HMC and flow comparisons")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -196,6 +201,10 @@ def main():
         sys.exit(2)
 
     gf = __graft_entry__.load_package()
+    if args.stout_only:  # profiling aid: only the stout-HMC and flow comparisons
+        print(json.dumps({"stout_hmc": stout_hmc_thermalized(gf, ntherm=args.stout_therm),
+                          "wilson_flow": flow_compare(gf, steps=30)}))
+        return
     V = int(np.prod(DIMS))
     log(f"[bench] 32^4 FP64 workload, V={V}")
     L = make_lattice(gf, DIMS, "f64")
@@ -251,7 +260,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(DIMS, 2)
+        cpu = cpu_baseline(DIMS)
         log(f"[bench] cpu {cpu}")
 
     out = {

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libgfhip.so")
 GF_F64, GF_F32 = 0, 1
 GF_OK, GF_EINVAL, GF_EDEVICE, GF_ENOMEM, GF_ESTATE = 0, -1, -2, -3, -4
 GF_QPQ, GF_PQP, GF_OMELYAN2MN, GF_FUSED = 0, 1, 2, 0x100
-GF_STOUT_STORE, GF_STOUT_RECOMPUTE = 0, 1
+GF_STOUT_STORE, GF_STOUT_RECOMPUTE, GF_STOUT_AUTO = 0, 1, 2
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -50,10 +50,15 @@ SIGNATURES = {
     "gf_flow": (ctypes.c_int, [_CTX, ctypes.c_double, ctypes.c_int, ctypes.c_int, _D]),
     "gf_energy_density": (ctypes.c_int, [_CTX, _D]),
     "gf_stout_config": (ctypes.c_int, [_CTX, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    "gf_stout_mode": (ctypes.c_int, [_CTX, _I]),
     "gf_stout_smear": (ctypes.c_int, [_CTX, _D]),
     "gf_stout_backward": (ctypes.c_int, [_CTX, _D, _D]),
     "gf_stout_force": (ctypes.c_int, [_CTX, ctypes.c_double]),
     "gf_stout_force_kick": (ctypes.c_int, [_CTX, ctypes.c_double, ctypes.c_double]),
+    "gf_ildg_info": (ctypes.c_int, [ctypes.c_char_p, _I, _I, ctypes.POINTER(ctypes.c_longlong)]),
+    "gf_ildg_read": (ctypes.c_int, [ctypes.c_char_p, _I, _D, _I]),
+    "gf_ildg_write": (ctypes.c_int, [ctypes.c_char_p, _I, _D, ctypes.c_int]),
+    "gf_ildg_last_error": (ctypes.c_char_p, []),
     "gf_event_record": (ctypes.c_int, [_CTX, ctypes.c_int]),
     "gf_event_elapsed": (ctypes.c_int, [_CTX, ctypes.c_int, ctypes.c_int, _F]),
 }
@@ -82,6 +87,14 @@ def load():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def check_ildg(code):
+    """Raise GFError with the ILDG layer's message for a non-zero status."""
+    if code != GF_OK:
+        msg = load().gf_ildg_last_error()
+        raise GFError(code, msg.decode() if msg else "")
+    return code
 
 
 def check(ctx, code):

@@ -16,6 +16,8 @@ are the same names without ``!``; they mutate their first argument in place):
                                                    docs/src/howtoimplement.md:94-112
   gradient_flow / flow       API.jl:420-437, gradientflow.jl:118-238
   stout_smearing / smear / back_prop   API.jl:452-480, Abstractsmearing.jl:237-411
+  ILDG / load_gaugefield / save_binarydata     ildg_format.jl:686-695, 797-833, 902-969
+  save_configuration / load_configuration_     API.jl:711-727, 781-826 (format=:ildg)
 
 Only what the hot path needs exists here: SU(3), four dimensions, periodic boundaries,
 the Wilson plaquette action (loop set plaquette ∪ plaquette^dag).  Everything runs
@@ -66,6 +68,11 @@ __all__ = [
     "smear",
     "back_prop",
     "SmearedGaugeAction",
+    "ILDG",
+    "load_gaugefield",
+    "save_binarydata",
+    "save_configuration",
+    "load_configuration_",
 ]
 
 
@@ -475,8 +482,8 @@ class StoutSmearing:
     def __init__(self, U, levels, rho, mode="store"):
         if levels < 0:
             raise ArgumentError("levels must be nonnegative")
-        if mode not in ("store", "recompute"):
-            raise ArgumentError("mode must be 'store' or 'recompute'")
+        if mode not in ("store", "recompute", "auto"):
+            raise ArgumentError("mode must be 'store', 'recompute' or 'auto'")
         self.levels = int(levels)
         self.rho = float(rho)
         self.mode = mode
@@ -485,7 +492,7 @@ class StoutSmearing:
         U.lattice.stout_config(self.levels, self.rho, self.mode)
 
 
-def stout_smearing(U, *, loops="plaquette", rho=0.1, levels=1, mode="store"):
+def stout_smearing(U, *, loops="plaquette", rho=0.1, levels=1, mode="auto"):
     """stout_smearing (API.jl:452-467): `levels` plaquette layers at the same rho"""
     names = [loops] if isinstance(loops, str) else list(loops)
     if names != ["plaquette"]:
@@ -530,3 +537,79 @@ class SmearedGaugeAction:
         U.lattice.stout_smear(download=False)
         # potential of the top smeared level with zero momenta contribution
         return U.lattice.hamiltonian(self.coefficient) - 0.5 * U.lattice.momenta_dot()
+
+
+# ----------------------------------------------------------------------------- ILDG files
+
+
+class ILDG:
+    """Header of an ILDG file (ILDG(filename), ildg_format.jl:686-695).
+
+    ``len(ildg) == 1``; ``ildg[1]`` (1-based, as in the reference) is a dict with "L",
+    "NC" and "precision".  A file without an ildg-format record reports L = None and the
+    reference's default precision 64 (load_gaugefield!, ildg_format.jl:919-923)."""
+
+    def __init__(self, filename):
+        from .ildg import ildg_info
+
+        self.filename = str(filename)
+        info = ildg_info(filename)
+        self.header = [{
+            "L": list(info["L"]) if info["L"] else None,
+            "NC": 3,
+            "precision": info["precision"] or 64,
+            "payload_bytes": info["payload_bytes"],
+        }]
+
+    def __len__(self):
+        return len(self.header)
+
+    def __getitem__(self, i):
+        if not 1 <= i <= len(self.header):
+            raise IndexError(f"ILDG index {i} out of range 1:{len(self.header)}")
+        return self.header[i - 1]
+
+
+def load_gaugefield(U, i, ildg, L, NC):
+    """load_gaugefield!(U, i, ildg, L, NC) (ildg_format.jl:902-969): read record i into U."""
+    L = [int(x) for x in L]
+    if list(U.dims) != L:
+        raise ArgumentError(f"lattice mismatch: U has {list(U.dims)} but L = {L}")
+    if NC != U.NC:
+        raise ArgumentError(f"NC mismatch: U has {U.NC} but NC = {NC}")
+    ildg[i]  # range check
+    U.lattice.load_ildg(ildg.filename)
+    return U
+
+
+def _ildg_precision(U, precision):
+    if precision == "field":
+        return 64 if U.eltype == np.complex128 else 32
+    if precision not in (32, 64):
+        raise ArgumentError(f"ILDG precision must be :field, 32, or 64, got {precision!r}")
+    return int(precision)
+
+
+def save_binarydata(U, filename, *, precision="field"):
+    """save_binarydata(U, filename; precision) (ildg_format.jl:797-833): one ILDG file
+    with an ildg-format record and the ildg-binary-data record."""
+    U.lattice.save_ildg(filename, _ildg_precision(U, precision))
+
+
+def save_configuration(filename, U, *, format="ildg", **kwargs):
+    """save_configuration (API.jl:711-727); the HIP backend writes ILDG only."""
+    if format == "ildg":
+        return save_binarydata(U, filename, **kwargs)
+    if format in ("jld2", "bridge"):
+        raise ArgumentError(f"format :{format} is not provided by the HIP backend; use :ildg")
+    raise ArgumentError(f"format must be :jld2, :bridge, or :ildg; got {format}")
+
+
+def load_configuration_(U, filename, *, format="ildg"):
+    """load_configuration!(U, filename; format) (API.jl:781-826); ILDG only here."""
+    if format == "ildg":
+        U.lattice.load_ildg(filename)
+        return U
+    if format in ("jld2", "bridge"):
+        raise ArgumentError(f"format :{format} is not provided by the HIP backend; use :ildg")
+    raise ArgumentError(f"format must be :jld2, :bridge, or :ildg; got {format}")

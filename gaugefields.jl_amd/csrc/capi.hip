@@ -613,7 +613,20 @@ int gf_stout_config(gf_ctx* c, int nlev, double rho, int mode) {
     CTXCHK(c);
     if (nlev < 0 || nlev > GF_MAX_LEV) return fail(c, GF_EINVAL, "nlev must be in 0..%d; got %d", GF_MAX_LEV, nlev);
     if (!std::isfinite(rho)) return fail(c, GF_EINVAL, "rho must be finite");
-    if (mode != GF_STOUT_STORE && mode != GF_STOUT_RECOMPUTE) return fail(c, GF_EINVAL, "unknown stout mode %d", mode);
+    if (mode != GF_STOUT_STORE && mode != GF_STOUT_RECOMPUTE && mode != GF_STOUT_AUTO)
+        return fail(c, GF_EINVAL, "unknown stout mode %d", mode);
+    if (mode == GF_STOUT_AUTO) {
+        // bytes still to allocate for levels, derivatives and (if stored) C per level
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
+        size_t need_levels = 0, need_c = 0;
+        for (int l = 0; l < nlev; l++) {
+            if (!c->lev[l]) need_levels += link_bytes(c);
+            if (!c->Cst[l]) need_c += link_bytes(c);
+        }
+        need_levels += (c->D ? 0 : link_bytes(c)) + (c->D2 ? 0 : link_bytes(c)) + (c->G ? 0 : link_bytes(c));
+        mode = (double)(need_levels + need_c) * 1.1 <= (double)free_b ? GF_STOUT_STORE : GF_STOUT_RECOMPUTE;
+    }
     // release stored C of a previous store-mode configuration if no longer needed
     for (int l = 0; l < GF_MAX_LEV; l++)
         if (l >= nlev || mode != GF_STOUT_STORE) release(c, &c->Cst[l], link_bytes(c));
@@ -622,6 +635,12 @@ int gf_stout_config(gf_ctx* c, int nlev, double rho, int mode) {
     c->rho = rho;
     c->smode = mode;
     if (nlev > 0) TRY(stout_alloc(c));
+    return GF_OK;
+}
+
+int gf_stout_mode(const gf_ctx* c, int* mode) {
+    if (!c || !mode) return GF_EINVAL;
+    *mode = c->smode;
     return GF_OK;
 }
 

@@ -3,12 +3,17 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <type_traits>
 
 #include "lattice.hpp"
 
 namespace gf {
 
 constexpr int kBlock = 256;
+
+// FP32-bulk kernels re-project every link they update onto SU(3)
+template <typename T>
+inline constexpr bool kFloatBulk = std::is_same<T, float>::value;
 
 inline int grid_for(int V, int block = kBlock) { return (V + block - 1) / block; }
 // grid of the (site, direction) kernels: one 256-thread workgroup per 64 sites

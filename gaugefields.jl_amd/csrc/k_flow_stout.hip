@@ -32,9 +32,10 @@ __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restric
 #pragma unroll
         for (int k = 0; k < 8; k++) c[k] = alpha * c[k];
     }
-    stp(A, V, mu, s, c);
-    const M3<T> E = expi_herm(herm_from_coeffs(c, T(1)));
-    stl(Wout, V, mu, s, mul(E, w));
+    stp<T, GF_ST_NT>(A, V, mu, s, c);
+    M3<T> wn = mul(expi_herm(herm_from_coeffs(c, T(1))), w);
+    if (kFloatBulk<T>) reunitarize(wn);
+    stl<T, GF_ST_NT>(Wout, V, mu, s, wn);
 }
 
 // ---------------------------------------------------------------------------
@@ -54,8 +55,8 @@ __global__ __launch_bounds__(kBlock) void k_stout_forward(Geo g, const T* __rest
     const M3<T> C = scal(rho, dag(staple_sigma(g, Uin, V, s, x, mu)));
     const M3<T> u = ldl(Uin, V, mu, s);
     const M3<T> E = exp_antiherm(ta_mat(mul_nd(C, u)));
-    stl(Uout, V, mu, s, mul(E, u));
-    if (STORE) stl(Cst, V, mu, s, C);
+    stl<T, GF_ST_NT>(Uout, V, mu, s, mul(E, u));
+    if (STORE) stl<T, GF_ST_NT>(Cst, V, mu, s, C);
 }
 
 // ---------------------------------------------------------------------------
@@ -80,14 +81,19 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
         decode(g, s, x);
         C = scal(rho, dag(staple_sigma(g, Uin, V, s, x, mu)));
     }
-    const M3<T> u = ldl(Uin, V, mu, s);
-    const M3<T> Q = ta_mat(mul_nd(C, u));
-    const M3<T> Do = ldl(Dout, V, mu, s);
-    M3<T> D = mul(Do, exp_antiherm(Q));
-    const M3<T> Lam = ta_mat(cdexpqdq(mul(u, Do), Q));
-    stl(G, V, mu, s, mul_dn(u, Lam));
+    M3<T> E, L;
+    {
+        const M3<T> u = ldl(Uin, V, mu, s);
+        exp_dexp(ta_mat(mul_nd(C, u)), mul(u, ldl(Dout, V, mu, s)), E, L);
+    }
+    const M3<T> Lam = ta_mat(L);
+    // re-read U_mu and Dout_mu (L1/L2 hits) instead of keeping them live across exp_dexp
+    int s1 = s;
+    asm volatile("" : "+v"(s1));
+    stl<T, GF_ST_NT>(G, V, mu, s, mul_dn(ldl(Uin, V, mu, s1), Lam));
+    M3<T> D = mul(ldl(Dout, V, mu, s1), E);
     acc_dag(D, mul(Lam, C));
-    stl(Din, V, mu, s, D);
+    stl<T, GF_ST_NT>(Din, V, mu, s, D);
 }
 
 // ---------------------------------------------------------------------------
@@ -113,24 +119,33 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* _
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             if (b == a) continue;
-            int y0 = y, y1 = ypa, ypb = y + dlt(g, x, b, 1);
+            const int ypb0 = y + dlt(g, x, b, 1);
             const int dmb = dlt(g, x, b, -1);
-            int ymb = y + dmb, ypamb = ypa + dmb;
-            asm volatile("" : "+v"(y0), "+v"(y1), "+v"(ypb), "+v"(ymb), "+v"(ypamb));  // no cross-plane merging
+            const int ymb0 = y + dmb, ypamb0 = ypa + dmb;
+            // each term group reloads its links through its own opaque index copies, so the
+            // compiler issues them group by group instead of hoisting all twelve link loads
             {  // U_b(y+a) [G_a(y+b) U_b(y)^dag + U_a(y+b)^dag G_b(y)]
+                int y0 = y, y1 = ypa, ypb = ypb0;
+                asm volatile("" : "+v"(y0), "+v"(y1), "+v"(ypb));
                 M3<T> t = mul_nd(ldl(G, V, a, ypb), ldl(U, V, b, y0));
                 acc_mul_dn(t, ldl(U, V, a, ypb), ldl(G, V, b, y0));
                 acc_mul(acc, ldl(U, V, b, y1), t);
             }
             {  // U_b(y-b+a)^dag [G_a(y-b) U_b(y-b) + (G_b(y-b) U_a(y-b))^dag]
+                int ymb = ymb0, ypamb = ypamb0;
+                asm volatile("" : "+v"(ymb), "+v"(ypamb));
                 M3<T> t = mul(ldl(G, V, a, ymb), ldl(U, V, b, ymb));
                 acc_dag(t, mul(ldl(G, V, b, ymb), ldl(U, V, a, ymb)));
                 acc_mul_dn(acc, ldl(U, V, b, ypamb), t);
             }
             {  // (U_b(y) U_a(y+b) G_b(y+a))^dag
+                int y0 = y, y1 = ypa, ypb = ypb0;
+                asm volatile("" : "+v"(y0), "+v"(y1), "+v"(ypb));
                 acc_dag(acc, mul(mul(ldl(U, V, b, y0), ldl(U, V, a, ypb)), ldl(G, V, b, y1)));
             }
             {  // G_b(y+a-b) U_a(y-b)^dag U_b(y-b)
+                int ymb = ymb0, ypamb = ypamb0;
+                asm volatile("" : "+v"(ymb), "+v"(ypamb));
                 acc_mul(acc, ldl(G, V, b, ypamb), mul_dn(ldl(U, V, a, ymb), ldl(U, V, b, ymb)));
             }
         }
@@ -140,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* _
             d.m[k].r = fma(rho, acc.m[k].r, d.m[k].r);
             d.m[k].i = fma(rho, acc.m[k].i, d.m[k].i);
         }
-        stl(Din, V, a, y, d);
+        stl<T, GF_ST_NT>(Din, V, a, y, d);
     }
 }
 

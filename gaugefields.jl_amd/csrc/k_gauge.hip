@@ -27,9 +27,10 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(Geo g, const T* __restrict
             const int mu = pl < 3 ? 0 : (pl < 5 ? 1 : 2);
             const int nu = pl < 3 ? pl + 1 : (pl < 5 ? pl - 1 : 3);
             const int spm = s + dlt(g, x, mu, 1), spn = s + dlt(g, x, nu, 1);
-            const M3<T> a = mul(ldl(U, V, mu, s), ldl(U, V, nu, spm));  // U_mu(x) U_nu(x+mu)
-            const M3<T> b = mul(ldl(U, V, nu, s), ldl(U, V, mu, spn));  // U_nu(x) U_mu(x+nu)
-            v += (double)retr_nd(a, b);
+            // products in FP64 also for FP32 fields: the action/Hamiltonian sums carry no float rounding
+            const M3<double> a = mul(to_f64(ldl(U, V, mu, s)), to_f64(ldl(U, V, nu, spm)));  // U_mu(x) U_nu(x+mu)
+            const M3<double> b = mul(to_f64(ldl(U, V, nu, s)), to_f64(ldl(U, V, mu, spn)));  // U_nu(x) U_mu(x+nu)
+            v += retr_nd(a, b);
         }
     }
     block_sum_store<kBlock>(v, partial);
@@ -94,16 +95,17 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U
     if (MODE == FORCE_STORE) {
 #pragma unroll
         for (int a = 0; a < 8; a++) c[a] *= fac;
-        stp(FP, V, mu, s, c);
+        stp<T, GF_ST_NT>(FP, V, mu, s, c);
     } else {
         T p[8];
         ldp(FP, V, mu, s, p);
 #pragma unroll
         for (int a = 0; a < 8; a++) p[a] = fma(fac, c[a], p[a]);
-        stp(FP, V, mu, s, p);
+        stp<T, GF_ST_NT>(FP, V, mu, s, p);
         if (MODE == FORCE_KICK_EXP) {
-            const M3<T> E = expi_herm(herm_from_coeffs(p, eps_u));
-            stl(Unew, V, mu, s, mul(E, umu));
+            M3<T> un = mul(expi_herm(herm_from_coeffs(p, eps_u)), umu);
+            if (kFloatBulk<T>) reunitarize(un);
+            stl<T, GF_ST_NT>(Unew, V, mu, s, un);
         }
     }
 }
@@ -112,13 +114,18 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_exp_update(Geo g, const T* __restrict__ Uin, T* __restrict__ Uout,
                                                        const T* __restrict__ P, T eps) {
-    const SiteMu sm = site_mu_index(g);
-    if (sm.s >= g.V) return;
+    // purely streaming: one thread per site walking its four links keeps more loads in flight
+    const int s = site_index();
+    if (s >= g.V) return;
     const unsigned V = g.V;
-    T p[8];
-    ldp(P, V, sm.mu, sm.s, p);
-    const M3<T> E = expi_herm(herm_from_coeffs(p, eps));
-    stl(Uout, V, sm.mu, sm.s, mul(E, ldl(Uin, V, sm.mu, sm.s)));
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) {
+        T p[8];
+        ldp(P, V, mu, s, p);
+        M3<T> un = mul(expi_herm(herm_from_coeffs(p, eps)), ldl(Uin, V, mu, s));
+        if (kFloatBulk<T>) reunitarize(un);
+        stl<T, GF_ST_NT>(Uout, V, mu, s, un);
+    }
 }
 
 // F_mu = -(1/3) TA(U_mu D_mu) from a matrix derivative D (stout force,
@@ -136,14 +143,14 @@ __global__ __launch_bounds__(kBlock) void k_force_from_deriv(Geo g, const T* __r
     if (MODE == FORCE_STORE) {
 #pragma unroll
         for (int a = 0; a < 8; a++) c[a] *= fac;
-        stp(FP, V, mu, s, c);
+        stp<T, GF_ST_NT>(FP, V, mu, s, c);
     } else {
         T p[8];
         ldp(FP, V, mu, s, p);
         const T f = eps * fac;
 #pragma unroll
         for (int a = 0; a < 8; a++) p[a] = fma(f, c[a], p[a]);
-        stp(FP, V, mu, s, p);
+        stp<T, GF_ST_NT>(FP, V, mu, s, p);
     }
 }
 
@@ -269,7 +276,7 @@ hipError_t launch_force(int mode, const Geo& g, const T* U, T* FP, T* Unew, doub
 }
 template <typename T>
 hipError_t launch_exp_update(const Geo& g, const T* Uin, T* Uout, const T* P, double eps, hipStream_t st) {
-    k_exp_update<T><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Uout, P, (T)eps);
+    k_exp_update<T><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, Uout, P, (T)eps);
     return hipGetLastError();
 }
 template <typename T>

@@ -118,13 +118,19 @@ __device__ __forceinline__ double2 buf_ld(__amdgpu_buffer_rsrc_t r, unsigned vof
 __device__ __forceinline__ float2 buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, const float*) {
     return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
 }
+// store cache policy: GF_ST_DEFAULT, or GF_ST_NT (non-temporal: streamed results that this
+// launch does not read back, so they do not evict the neighbour links held in L2)
+#define GF_ST_DEFAULT 0
+#define GF_ST_NT 2
+template <int AUX = GF_ST_DEFAULT>
 __device__ __forceinline__ void buf_st(double2 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(HIP_vector_type<unsigned, 4>::Native_vec_, v), r,
-                                           (int)voff, (int)soff, 0);
+                                           (int)voff, (int)soff, AUX);
 }
+template <int AUX = GF_ST_DEFAULT>
 __device__ __forceinline__ void buf_st(float2 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(HIP_vector_type<unsigned, 2>::Native_vec_, v), r,
-                                          (int)voff, (int)soff, 0);
+                                          (int)voff, (int)soff, AUX);
 }
 
 // link field: plane (mu*9 + k) holds complex element k = 3*i + j of U_mu for all sites
@@ -142,7 +148,7 @@ __device__ __forceinline__ M3<T> ldl(const T* __restrict__ p, unsigned V, int mu
     return m;
 }
 
-template <typename T>
+template <typename T, int AUX = GF_ST_DEFAULT>
 __device__ __forceinline__ void stl(T* __restrict__ p, unsigned V, int mu, unsigned s, const M3<T>& m) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 9u * mu, 9u);
     const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
@@ -151,7 +157,7 @@ __device__ __forceinline__ void stl(T* __restrict__ p, unsigned V, int mu, unsig
         T2<T> v;
         v.x = m.m[k].r;
         v.y = m.m[k].i;
-        buf_st(v, r, off, k * ps);
+        buf_st<AUX>(v, r, off, k * ps);
     }
 }
 
@@ -168,7 +174,7 @@ __device__ __forceinline__ void ldp(const T* __restrict__ p, unsigned V, int mu,
     }
 }
 
-template <typename T>
+template <typename T, int AUX = GF_ST_DEFAULT>
 __device__ __forceinline__ void stp(T* __restrict__ p, unsigned V, int mu, unsigned s, const T c[8]) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 4u * mu, 4u);
     const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
@@ -177,7 +183,7 @@ __device__ __forceinline__ void stp(T* __restrict__ p, unsigned V, int mu, unsig
         T2<T> v;
         v.x = c[2 * j];
         v.y = c[2 * j + 1];
-        buf_st(v, r, off, j * ps);
+        buf_st<AUX>(v, r, off, j * ps);
     }
 }
 
@@ -197,19 +203,22 @@ __device__ __forceinline__ M3<T> staple_sigma(const Geo& g, const T* __restrict_
 #pragma unroll
     for (int nu = 0; nu < 4; nu++) {
         if (nu == mu) continue;
-        int s0 = s, s1 = spm, spn = s + dlt(g, x, nu, 1);
-        const int dmn = dlt(g, x, nu, -1);
-        int smn = s + dmn, spmmn = spm + dmn;
         // opaque copies of the site indices: stops the compiler from merging these link
-        // loads with identical ones of other (mu, nu) iterations, which would keep dozens
-        // of 3x3 matrices live across the whole kernel and spill
-        asm volatile("" : "+v"(s0), "+v"(s1), "+v"(spn), "+v"(smn), "+v"(spmmn));
-        // upper: (U_nu(x+mu) U_mu(x+nu)^dag) U_nu(x)^dag
-        const M3<T> a = mul_nd(ldl(U, V, nu, s1), ldl(U, V, mu, spn));
-        acc_mul_nd(sig, a, ldl(U, V, nu, s0));
-        // lower: (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
-        const M3<T> b = mul(ldl(U, V, mu, smn), ldl(U, V, nu, spmmn));
-        acc_mul_dn(sig, b, ldl(U, V, nu, smn));
+        // loads with identical ones of other (mu, nu) iterations (or hoisting them all
+        // ahead), which would keep dozens of 3x3 matrices live and spill
+        {  // upper: (U_nu(x+mu) U_mu(x+nu)^dag) U_nu(x)^dag
+            int s0 = s, s1 = spm, spn = s + dlt(g, x, nu, 1);
+            asm volatile("" : "+v"(s0), "+v"(s1), "+v"(spn));
+            const M3<T> a = mul_nd(ldl(U, V, nu, s1), ldl(U, V, mu, spn));
+            acc_mul_nd(sig, a, ldl(U, V, nu, s0));
+        }
+        {  // lower: (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
+            const int dmn = dlt(g, x, nu, -1);
+            int smn = s + dmn, spmmn = spm + dmn;
+            asm volatile("" : "+v"(smn), "+v"(spmmn));
+            const M3<T> b = mul(ldl(U, V, mu, smn), ldl(U, V, nu, spmmn));
+            acc_mul_dn(sig, b, ldl(U, V, nu, smn));
+        }
     }
     return sig;
 }

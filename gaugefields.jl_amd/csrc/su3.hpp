@@ -171,6 +171,81 @@ __device__ __forceinline__ T retr_nd(const M3<T>& a, const M3<T>& b) {
     return s;
 }
 
+template <typename T>
+__device__ __forceinline__ M3<double> to_f64(const M3<T>& a) {
+    M3<double> c;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        c.m[k].r = (double)a.m[k].r;
+        c.m[k].i = (double)a.m[k].i;
+    }
+    return c;
+}
+
+// Projection back to SU(3) as normalize3! + m3complv3! do (AbstractGaugefields.jl:3378-3457):
+// Gram-Schmidt on rows 1 and 2, row 3 = conj(row1 x row2).  The FP32-bulk kernels apply it to
+// every link they update so that float rounding cannot accumulate into a unitarity drift.  The
+// projection itself runs in double and only its result is rounded: a float Gram-Schmidt leaves
+// the row norms about 1e-7 too large on average (1/sqrt rounds asymmetrically next to 1), a bias
+// that shows up directly in the plaquette.
+__device__ __forceinline__ void reunitarize_f64(M3<double>& u) {
+    cx<double> r1[3], r2[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        r1[c] = u.m[c];
+        r2[c] = u.m[3 + c];
+    }
+    double w2 = 0.0, w1r = 0.0, w1i = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        w2 += r1[c].r * r1[c].r + r1[c].i * r1[c].i;
+        w1r += r2[c].r * r1[c].r + r2[c].i * r1[c].i;  // row2 . conj(row1)
+        w1i += r2[c].i * r1[c].r - r2[c].r * r1[c].i;
+    }
+    const double inv2 = 1.0 / w2;
+    w1r = -w1r * inv2;
+    w1i = -w1i * inv2;
+    double w3 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double xr = r2[c].r + w1r * r1[c].r - w1i * r1[c].i;
+        const double xi = r2[c].i + w1r * r1[c].i + w1i * r1[c].r;
+        r2[c].r = xr;
+        r2[c].i = xi;
+        w3 += xr * xr + xi * xi;
+    }
+    const double n1 = 1.0 / sqrt(w2), n2 = 1.0 / sqrt(w3);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        r1[c].r *= n1;
+        r1[c].i *= n1;
+        r2[c].r *= n2;
+        r2[c].i *= n2;
+        u.m[c] = r1[c];
+        u.m[3 + c] = r2[c];
+    }
+    // row3_k = conj(row1_{k+1} row2_{k+2} - row1_{k+2} row2_{k+1})
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int a = (k + 1) % 3, b = (k + 2) % 3;
+        const double re = r1[a].r * r2[b].r - r1[a].i * r2[b].i - (r1[b].r * r2[a].r - r1[b].i * r2[a].i);
+        const double im = r1[a].r * r2[b].i + r1[a].i * r2[b].r - (r1[b].r * r2[a].i + r1[b].i * r2[a].r);
+        u.m[6 + k].r = re;
+        u.m[6 + k].i = -im;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void reunitarize(M3<T>& u) {
+    M3<double> d = to_f64(u);
+    reunitarize_f64(d);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        u.m[k].r = (T)d.m[k].r;
+        u.m[k].i = (T)d.m[k].i;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // su(3) projections (basis i lambda_a / 2, order of TA_gaugefields_4D_serial.jl:243-260)
 // ---------------------------------------------------------------------------
@@ -359,8 +434,8 @@ __device__ __forceinline__ void ch_coeffs(double c0, double c1, dc f[3], dc b1[3
 // c1 below which exp(iH) = 1 + iH - H^2/2 (error < 1e-30)
 #define GF_CH_EXP_TINY 1e-20
 // c1 below which C star dexp(Q)/dQ is summed from its power series
-#define GF_CH_DER_SERIES 1e-2
-#define GF_CH_DER_TERMS 12
+#define GF_CH_DER_SERIES 1e-3
+#define GF_CH_DER_TERMS 9
 
 // exp(iH) for Hermitian traceless H
 template <typename T>
@@ -483,6 +558,113 @@ __device__ __forceinline__ M3<T> cdexpqdq(const M3<T>& C, const M3<T>& Q) {
         Lh.m[k].i = -r;
     }
     return Lh;
+}
+
+// exp(Q) and L = X star d exp(Q)/dQ together, sharing one evaluation of the invariants and
+// Cayley-Hamilton coefficients (stout backward: E = e^{Q_mu}, L = CdexpQdQ(U_mu Dout_mu, Q_mu))
+template <typename T>
+__device__ __forceinline__ void exp_dexp(const M3<T>& Q, const M3<T>& X, M3<T>& E, M3<T>& L) {
+    M3<T> H;  // H = -iQ (Hermitian)
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        H.m[k].r = Q.m[k].i;
+        H.m[k].i = -Q.m[k].r;
+    }
+    double c0, c1;
+    herm_invariants(H, c0, c1);
+    const M3<T> H2 = mul(H, H);
+    if (c1 < GF_CH_DER_SERIES) {
+        if (c1 < GF_CH_EXP_TINY) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                E.m[k].r = -H.m[k].i - T(0.5) * H2.m[k].r;
+                E.m[k].i = H.m[k].r - T(0.5) * H2.m[k].i;
+            }
+            E.m[0].r += T(1);
+            E.m[4].r += T(1);
+            E.m[8].r += T(1);
+        } else {
+            dc f[3];
+            ch_coeffs<false>(c0, c1, f, nullptr, nullptr);
+            const T f0r = T(f[0].r), f0i = T(f[0].i), f1r = T(f[1].r), f1i = T(f[1].i);
+            const T f2r = T(f[2].r), f2i = T(f[2].i);
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                E.m[k].r = f1r * H.m[k].r - f1i * H.m[k].i + f2r * H2.m[k].r - f2i * H2.m[k].i;
+                E.m[k].i = f1r * H.m[k].i + f1i * H.m[k].r + f2r * H2.m[k].i + f2i * H2.m[k].r;
+            }
+            E.m[0].r += f0r;
+            E.m[0].i += f0i;
+            E.m[4].r += f0r;
+            E.m[4].i += f0i;
+            E.m[8].r += f0r;
+            E.m[8].i += f0i;
+        }
+        // L = sum_{n>=1} 1/n! sum_{k<n} Q^(n-1-k) X Q^k
+        M3<T> Ln = X, XQn = X;
+        L = X;
+        T fact = T(1);
+#pragma unroll 1
+        for (int n = 2; n <= GF_CH_DER_TERMS; n++) {
+            XQn = mul(XQn, Q);
+            M3<T> t = mul(Q, Ln);
+            acc(t, XQn);
+            Ln = t;
+            fact *= T(n);
+            const T inv = T(1) / fact;
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                L.m[k].r = fma(inv, Ln.m[k].r, L.m[k].r);
+                L.m[k].i = fma(inv, Ln.m[k].i, L.m[k].i);
+            }
+        }
+        return;
+    }
+    dc f[3], b1[3], b2[3];
+    ch_coeffs<true>(c0, c1, f, b1, b2);
+    const T f0r = T(f[0].r), f0i = T(f[0].i), f1r = T(f[1].r), f1i = T(f[1].i);
+    const T f2r = T(f[2].r), f2i = T(f[2].i);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        E.m[k].r = f1r * H.m[k].r - f1i * H.m[k].i + f2r * H2.m[k].r - f2i * H2.m[k].i;
+        E.m[k].i = f1r * H.m[k].i + f1i * H.m[k].r + f2r * H2.m[k].i + f2i * H2.m[k].r;
+    }
+    E.m[0].r += f0r;
+    E.m[0].i += f0i;
+    E.m[4].r += f0r;
+    E.m[4].i += f0i;
+    E.m[8].r += f0r;
+    E.m[8].i += f0i;
+    dc t0 = {0.0, 0.0}, t1 = {0.0, 0.0}, t2 = {0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        t0.r += double(X.m[4 * i].r);
+        t0.i += double(X.m[4 * i].i);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const cx<T> c = X.m[3 * i + k];
+            const cx<T> h = H.m[3 * k + i];
+            const cx<T> g = H2.m[3 * k + i];
+            t1.r += double(c.r) * double(h.r) - double(c.i) * double(h.i);
+            t1.i += double(c.r) * double(h.i) + double(c.i) * double(h.r);
+            t2.r += double(c.r) * double(g.r) - double(c.i) * double(g.i);
+            t2.i += double(c.r) * double(g.i) + double(c.i) * double(g.r);
+        }
+    }
+    const dc tB1 = dca(dca(dcm(b1[0], t0), dcm(b1[1], t1)), dcm(b1[2], t2));
+    const dc tB2 = dca(dca(dcm(b2[0], t0), dcm(b2[1], t1)), dcm(b2[2], t2));
+    const T a1r = T(tB1.r), a1i = T(tB1.i), a2r = T(tB2.r), a2i = T(tB2.i);
+    M3<T> HC = mul(H, X);
+    acc_mul(HC, X, H);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        T r = a1r * H.m[k].r - a1i * H.m[k].i + a2r * H2.m[k].r - a2i * H2.m[k].i + f1r * X.m[k].r -
+              f1i * X.m[k].i + f2r * HC.m[k].r - f2i * HC.m[k].i;
+        T im = a1r * H.m[k].i + a1i * H.m[k].r + a2r * H2.m[k].i + a2i * H2.m[k].r + f1r * X.m[k].i +
+               f1i * X.m[k].r + f2r * HC.m[k].i + f2i * HC.m[k].r;
+        L.m[k].r = im;
+        L.m[k].i = -r;
+    }
 }
 
 }  // namespace gf

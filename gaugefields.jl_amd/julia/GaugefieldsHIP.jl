@@ -121,4 +121,26 @@ stout_config!(U::HIPConfiguration, nlev::Integer, rho::Real; recompute::Bool=fal
     _check(U, ccall((:gf_stout_config, libgfhip), Cint, (Ptr{Cvoid}, Cint, Cdouble, Cint),
                     U.ctx, Cint(nlev), rho, Cint(recompute ? 1 : 0)))
 
+# ILDG files (ildg_format.jl:686-969): host I/O in the library, no CLIME executables
+function _check_ildg(rc::Cint)
+    rc == 0 || throw(ArgumentError(unsafe_string(ccall((:gf_ildg_last_error, libgfhip), Cstring, ()))))
+    return nothing
+end
+
+function load_gaugefield!(U::HIPConfiguration, filename::AbstractString)
+    buf = Vector{ComplexF64}(undef, 4 * 9 * prod(U.dims))
+    prec = Ref{Cint}(0)
+    _check_ildg(ccall((:gf_ildg_read, libgfhip), Cint, (Cstring, Ptr{Cint}, Ptr{ComplexF64}, Ref{Cint}),
+                      filename, Cint[U.dims...], buf, prec))
+    _check(U, ccall((:gf_upload_links, libgfhip), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}), U.ctx, buf))
+    return Int(prec[])
+end
+
+function save_binarydata(U::HIPConfiguration, filename::AbstractString; precision::Integer=64)
+    buf = Vector{ComplexF64}(undef, 4 * 9 * prod(U.dims))
+    _check(U, ccall((:gf_download_links, libgfhip), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}), U.ctx, buf))
+    _check_ildg(ccall((:gf_ildg_write, libgfhip), Cint, (Cstring, Ptr{Cint}, Ptr{ComplexF64}, Cint),
+                      filename, Cint[U.dims...], buf, Cint(precision)))
+end
+
 end # module

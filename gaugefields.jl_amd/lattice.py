@@ -117,6 +117,20 @@ class Lattice:
     def restore_links(self):
         self._c(self._L.gf_restore_links(self._ctx))
 
+    def load_ildg(self, path):
+        """Read an ILDG file of this lattice's extents onto the device; returns its precision."""
+        from .ildg import read_ildg
+
+        U, prec = read_ildg(path, self.dims)
+        self.upload_links(U)
+        return prec
+
+    def save_ildg(self, path, precision=64):
+        """Write the device links to an ILDG file (precision 32 or 64 bits)."""
+        from .ildg import write_ildg
+
+        write_ildg(path, self.dims, self.download_links(), precision)
+
     def links_from_algebra(self, omega):
         """U = exp(i sum_a omega_a lambda_a / 2) per link (the configs' seeded start)."""
         self.set_cold()
@@ -187,9 +201,15 @@ class Lattice:
 
     # -- stout ----------------------------------------------------------------------------
     def stout_config(self, nlev, rho, mode="store"):
-        m = {"store": _lib.GF_STOUT_STORE, "recompute": _lib.GF_STOUT_RECOMPUTE}[mode]
+        """mode: "store" (keep C per level), "recompute", or "auto" (store if it fits)"""
+        m = {"store": _lib.GF_STOUT_STORE, "recompute": _lib.GF_STOUT_RECOMPUTE, "auto": _lib.GF_STOUT_AUTO}[mode]
         self._c(self._L.gf_stout_config(self._ctx, int(nlev), ctypes.c_double(rho), m))
         self.nlev = int(nlev)
+
+    def stout_mode(self):
+        m = ctypes.c_int()
+        self._c(self._L.gf_stout_mode(self._ctx, ctypes.byref(m)))
+        return {_lib.GF_STOUT_STORE: "store", _lib.GF_STOUT_RECOMPUTE: "recompute"}[m.value]
 
     def stout_smear(self, download=True):
         out = np.empty((4, self.V, 9), np.complex128) if download else None

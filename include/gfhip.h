@@ -37,7 +37,7 @@ typedef struct gf_ctx gf_ctx;
 enum { GF_F64 = 0, GF_F32 = 1 };
 enum { GF_OK = 0, GF_EINVAL = -1, GF_EDEVICE = -2, GF_ENOMEM = -3, GF_ESTATE = -4 };
 enum { GF_QPQ = 0, GF_PQP = 1, GF_OMELYAN2MN = 2, GF_FUSED = 0x100 };
-enum { GF_STOUT_STORE = 0, GF_STOUT_RECOMPUTE = 1 };
+enum { GF_STOUT_STORE = 0, GF_STOUT_RECOMPUTE = 1, GF_STOUT_AUTO = 2 };
 
 /* ---- lifecycle ------------------------------------------------------------------- */
 int gf_version(void);
@@ -112,8 +112,12 @@ int gf_energy_density(gf_ctx *ctx, double *E);
 /* ---- stout smearing ---------------------------------------------------------------- */
 /* nlev stout layers at rho (STOUT_Layer + CovNeuralnet, stout_fast.jl:6-122,
  * Abstractsmearing.jl:61-99); mode GF_STOUT_STORE keeps C_mu per level, GF_STOUT_RECOMPUTE
- * recomputes it in the backward pass.  nlev = 0 disables smearing. */
+ * recomputes it in the backward pass, GF_STOUT_AUTO stores when the device has room for the
+ * nlev extra link fields (plus a 10 % margin) and recomputes otherwise.  The smeared levels
+ * themselves are always kept.  nlev = 0 disables smearing. */
 int gf_stout_config(gf_ctx *ctx, int nlev, double rho, int mode);
+/* the memory mode in effect after gf_stout_config (GF_STOUT_STORE or GF_STOUT_RECOMPUTE) */
+int gf_stout_mode(const gf_ctx *ctx, int *mode);
 /* calc_smearedU (Abstractsmearing.jl:237-272): smeared links (all levels kept on device);
  * top level written to ref if ref != NULL */
 int gf_stout_smear(gf_ctx *ctx, double *ref);
@@ -124,6 +128,22 @@ int gf_stout_backward(gf_ctx *ctx, const double *dout_ref, double *din_ref);
  * links, back_prop, F = -(1/3) TA(U dSdU_bare) */
 int gf_stout_force(gf_ctx *ctx, double coeff);
 int gf_stout_force_kick(gf_ctx *ctx, double coeff, double eps);
+
+/* ---- ILDG / LIME configuration files (host only, no context) ----------------------- */
+/* ILDG(filename) header parse (ildg_format.jl:686, read_header :1049-1053): extents lx..lt
+ * and precision from the ildg-format record (zeros where the file has none) and the size of
+ * the ildg-binary-data payload in bytes. */
+int gf_ildg_info(const char *path, int *dims, int *precision_bits, long long *payload_bytes);
+/* load_gaugefield! (ildg_format.jl:902-969) + load_binarydata! (:862-887): the SU(3)
+ * configuration of extents dims into ref (link layout above).  Precision 32 or 64 bits from
+ * the ildg-format record, else from the payload size; *precision_bits reports it. */
+int gf_ildg_read(const char *path, const int *dims, double *ref, int *precision_bits);
+/* save_binarydata (ildg_format.jl:797-833) + _save_binarydata (:697-746) + pack_ildg_file
+ * (:50-65): an ildg-format record and the ildg-binary-data record, precision 32 or 64
+ * (anything else: GF_EINVAL, the reference's ArgumentError). */
+int gf_ildg_write(const char *path, const int *dims, const double *ref, int precision_bits);
+/* message of the last failing gf_ildg_* call on this thread */
+const char *gf_ildg_last_error(void);
 
 /* ---- timing (HIP events on the context's stream) ----------------------------------- */
 int gf_event_record(gf_ctx *ctx, int slot); /* slot 0..15 */

@@ -321,8 +321,8 @@ static void ch_coeffs(m3 Q, cplx f[3], cplx b[2][3]) {
 /* c1 below which exp(iQ) uses 1 + iQ - Q^2/2 (the formulas divide by 9u^2-w^2 ~ 2 c1) */
 #define CH_EXP_TINY 1e-20
 /* c1 below which C star dexp(Q)/dQ uses its power series (the b_ij formulas cancel) */
-#define CH_DER_SERIES 1e-2
-#define CH_DER_TERMS 12
+#define CH_DER_SERIES 1e-3
+#define CH_DER_TERMS 9
 
 /* exp(iH) for Hermitian traceless H */
 static m3 expi_herm(m3 H) {

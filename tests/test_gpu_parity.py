@@ -291,3 +291,14 @@ def test_morton_path_stout(Lat, oracle):
     L.force(COEFF)
     F = oracle.stout_force(dims, U, 3, 0.1, COEFF)
     assert maxdiff(L.download_force(), F) < 1e-12 * np.abs(F).max()
+
+
+def test_stout_auto_mode_stores_when_it_fits(Lat, oracle):
+    L = Lat()
+    L.stout_config(2, 0.1, "auto")
+    assert L.stout_mode() == "store"
+    U = alg_links(oracle, D4, 0.3, 64)
+    L.upload_links(U)
+    L.force(COEFF)
+    F = oracle.stout_force(D4, U, 2, 0.1, COEFF)
+    assert maxdiff(L.download_force(), F) < 1e-12 * np.abs(F).max()
