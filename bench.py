@@ -26,6 +26,9 @@ sys.path.insert(0, ROOT)
 import __graft_entry__  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# compulsory bytes per site, FP64, device layout (12-real links) and reference layout (18-real)
+ALG_BYTES = {"md_update_fused": 1280, "force_kick": 896, "exp_update": 1024, "plaquette": 384}
+REF_BYTES = {"md_update_fused": 1664, "force_kick": 1088, "exp_update": 1408, "plaquette": 576}
 DIMS = (32, 32, 32, 32)
 BETA = 6.0
 COEFF = BETA / 2
@@ -224,8 +227,13 @@ def main():
     ms_per_step = ms_ev / args.steps
     log(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step (wall {wall*1e3/args.steps:.4f})")
 
-    # algorithmic bytes per site: U in 72, P in 32, P out 32, U out 72 reals
-    bytes_md = 208 * 8 * V
+    # Algorithmic (compulsory) bytes per site of the device layout, each input read once and
+    # each output written once: an SU(3) link is stored as its first two rows (12 reals; the
+    # third is rebuilt as conj(row0 x row1)), a momentum as 8 reals.  Fused MD update: U in 48,
+    # P in 32, P out 32, U out 48 reals.  With the reference's 18-real links the same step
+    # would move 208 reals (1664 B) per site ("reference_bytes_per_site").
+    B = ALG_BYTES
+    bytes_md = B["md_update_fused"] * V
     achieved = bytes_md / (ms_per_step * 1e-3) / 1e9
     # companion kernels
     ms_force = time_kernel(L, lambda: L.force_kick(COEFF, 0.0), 10)
@@ -234,20 +242,21 @@ def main():
     # device-to-device copy of the link field (hipMemcpyAsync): the measured stream-copy
     # rate, and the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters
     ms_copy = time_kernel(L, L.save_links, 10)
-    copy_gbps = 2 * 72 * 8 * V / (ms_copy * 1e-3) / 1e9
+    copy_bytes = 2 * 48 * 8 * V  # read + write of one link field (12 reals per link)
+    copy_gbps = copy_bytes / (ms_copy * 1e-3) / 1e9
     plaq = L.plaquette()
     L.close()
+    def kern(name, ms, key="ms"):
+        return {key: ms, "bytes_per_site": B[name], "reference_bytes_per_site": REF_BYTES[name],
+                "GBps": B[name] * V / (ms * 1e-3) / 1e9}
+
     kernels = {
-        "md_update_fused": {"ms": ms_per_step, "bytes_per_site": 1664,
-                            "GBps": achieved},
-        "force_kick": {"ms": ms_force, "bytes_per_site": 1088,
-                       "GBps": 1088 * V / (ms_force * 1e-3) / 1e9},
-        "exp_update": {"ms": ms_exp, "bytes_per_site": 1408,
-                       "GBps": 1408 * V / (ms_exp * 1e-3) / 1e9},
-        "plaquette": {"ms_incl_readback": ms_plaq, "bytes_per_site": 576,
-                      "GBps": 576 * V / (ms_plaq * 1e-3) / 1e9},
+        "md_update_fused": kern("md_update_fused", ms_per_step),
+        "force_kick": kern("force_kick", ms_force),
+        "exp_update": kern("exp_update", ms_exp),
+        "plaquette": kern("plaquette", ms_plaq, "ms_incl_readback"),
     }
-    kernels["link_copy"] = {"ms": ms_copy, "bytes": 2 * 72 * 8 * V, "GBps": copy_gbps}
+    kernels["link_copy"] = {"ms": ms_copy, "bytes": copy_bytes, "GBps": copy_gbps}
     log(f"[bench] kernels {json.dumps(kernels)}")
 
     stout = None

@@ -13,6 +13,8 @@ GF_F64, GF_F32 = 0, 1
 GF_OK, GF_EINVAL, GF_EDEVICE, GF_ENOMEM, GF_ESTATE = 0, -1, -2, -3, -4
 GF_QPQ, GF_PQP, GF_OMELYAN2MN, GF_FUSED = 0, 1, 2, 0x100
 GF_STOUT_STORE, GF_STOUT_RECOMPUTE, GF_STOUT_AUTO = 0, 1, 2
+GF_TOPO_PLAQUETTE, GF_TOPO_CLOVER, GF_TOPO_IMPROVED, GF_TOPO_RECTANGLE = 0, 1, 2, 3
+GF_FLOW_NOBS = 7
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -36,6 +38,8 @@ SIGNATURES = {
     "gf_download_force": (ctypes.c_int, [_CTX, _D]),
     "gf_save_links": (ctypes.c_int, [_CTX]),
     "gf_restore_links": (ctypes.c_int, [_CTX]),
+    "gf_gaussian_momenta": (ctypes.c_int, [_CTX, ctypes.c_double, ctypes.c_ulonglong, ctypes.c_ulonglong]),
+    "gf_hot_start": (ctypes.c_int, [_CTX, ctypes.c_ulonglong]),
     "gf_plaquette": (ctypes.c_int, [_CTX, _D]),
     "gf_action": (ctypes.c_int, [_CTX, ctypes.c_double, _D]),
     "gf_momenta_dot": (ctypes.c_int, [_CTX, _D]),
@@ -49,6 +53,10 @@ SIGNATURES = {
                                         ctypes.c_double, _D]),
     "gf_flow": (ctypes.c_int, [_CTX, ctypes.c_double, ctypes.c_int, ctypes.c_int, _D]),
     "gf_energy_density": (ctypes.c_int, [_CTX, _D]),
+    "gf_topological_charge": (ctypes.c_int, [_CTX, ctypes.c_int, _D]),
+    "gf_topological_charge_density": (ctypes.c_int, [_CTX, ctypes.c_int, _D]),
+    "gf_energy_density_clover": (ctypes.c_int, [_CTX, _D]),
+    "gf_flow_measure": (ctypes.c_int, [_CTX, ctypes.c_double, ctypes.c_int, ctypes.c_int, _D]),
     "gf_stout_config": (ctypes.c_int, [_CTX, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
     "gf_stout_mode": (ctypes.c_int, [_CTX, _I]),
     "gf_stout_smear": (ctypes.c_int, [_CTX, _D]),

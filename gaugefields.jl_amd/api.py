@@ -16,6 +16,8 @@ are the same names without ``!``; they mutate their first argument in place):
                                                    docs/src/howtoimplement.md:94-112
   gradient_flow / flow       API.jl:420-437, gradientflow.jl:118-238
   stout_smearing / smear / back_prop   API.jl:452-480, Abstractsmearing.jl:237-411
+  topological_charge(_density)  AbstractGaugefields.jl:1430-1471
+  calculate_energy_density   samples/measurements/energydensity.jl:32-76 (clover)
   ILDG / load_gaugefield / save_binarydata     ildg_format.jl:686-695, 797-833, 902-969
   save_configuration / load_configuration_     API.jl:711-727, 781-826 (format=:ildg)
 
@@ -68,6 +70,9 @@ __all__ = [
     "smear",
     "back_prop",
     "SmearedGaugeAction",
+    "topological_charge",
+    "topological_charge_density",
+    "calculate_energy_density",
     "ILDG",
     "load_gaugefield",
     "save_binarydata",
@@ -613,3 +618,28 @@ def load_configuration_(U, filename, *, format="ildg"):
     if format in ("jld2", "bridge"):
         raise ArgumentError(f"format :{format} is not provided by the HIP backend; use :ildg")
     raise ArgumentError(f"format must be :jld2, :bridge, or :ildg; got {format}")
+
+
+# ----------------------------------------------------------------------------- field strengths
+
+_TOPO_PUBLIC = ("plaquette", "clover", "improved")
+
+
+def topological_charge(U, *, method="plaquette"):
+    """topological_charge(U; method) (AbstractGaugefields.jl:1452-1471)"""
+    if method not in _TOPO_PUBLIC:
+        raise ArgumentError("supported topological_charge methods are :plaquette, :clover, and :improved")
+    return U.lattice.topological_charge(method)
+
+
+def topological_charge_density(U, *, method="plaquette"):
+    """topological_charge_density(U; method) (AbstractGaugefields.jl:1430-1446): an
+    (NX, NY, NZ, NT) array indexed like the reference's q[ix, iy, iz, it]"""
+    if method not in _TOPO_PUBLIC:
+        raise ArgumentError("supported topological_charge_density methods are :plaquette, :clover, and :improved")
+    return U.lattice.topological_charge_density(method).reshape(U.dims, order="F")
+
+
+def calculate_energy_density(U):
+    """clover energy density (calculate_energy_density, samples/measurements/energydensity.jl:68-76)"""
+    return U.lattice.energy_density_clover()

@@ -76,7 +76,9 @@ int fail(gf_ctx* c, int code, const char* fmt, ...) {
         if (_e != hipSuccess) return fail((c), GF_EDEVICE, "hipSetDevice: %s", hipGetErrorString(_e));   \
     } while (0)
 
-size_t link_bytes(const gf_ctx* c) { return (size_t)72 * c->V * c->esz; }
+// SU(3) link field: 4 directions x rows 0-1 (12 reals); general matrix field: 4 x 18 reals
+size_t link_bytes(const gf_ctx* c) { return (size_t)48 * c->V * c->esz; }
+size_t mat_bytes(const gf_ctx* c) { return (size_t)72 * c->V * c->esz; }
 size_t mom_bytes(const gf_ctx* c) { return (size_t)32 * c->V * c->esz; }
 
 int alloc(gf_ctx* c, void** p, size_t n) {
@@ -129,15 +131,59 @@ int plaq_of(gf_ctx* c, void* U, double* sum) {
 
 void swap_links(gf_ctx* c) { std::swap(c->U, c->Ualt); }
 
+// field-strength observables of the current links: out = {clover energy sum, Q plaquette,
+// Q clover, Q rectangle}; dens_host (V doubles) receives the density of dens_method if non-null
+int fstrength(gf_ctx* c, int sets, double out[4], double* dens_host, int dens_method) {
+    double* dens = nullptr;
+    if (dens_host) {
+        TRY(need_staging(c));
+        dens = c->staging;
+    }
+    TRY(dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_field_strength<T>(c->geo, (const T*)c->U, sets, c->partial, c->dres, dens, dens_method,
+                                           c->stream));
+        return GF_OK;
+    }));
+    HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (dens_host)
+        HIPCHK(c, hipMemcpyAsync(dens_host, dens, sizeof(double) * (size_t)c->V, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 4; k++) out[k] = c->hres[k];
+    return GF_OK;
+}
+
+// loop sets a topological-charge method needs (bit 0 plaquette, 1 clover, 2 rectangle)
+int topo_sets(gf_ctx* c, int method, int* sets) {
+    switch (method) {
+        case GF_TOPO_PLAQUETTE: *sets = 1; return GF_OK;
+        case GF_TOPO_CLOVER: *sets = 2; return GF_OK;
+        case GF_TOPO_IMPROVED: *sets = 6; return GF_OK;
+        case GF_TOPO_RECTANGLE: *sets = 4; return GF_OK;
+        default:
+            return fail(c, GF_EINVAL,
+                        "supported topological_charge methods are :plaquette, :clover, and :improved; got %d", method);
+    }
+}
+
+double topo_of(int method, const double o[4]) {
+    switch (method) {
+        case GF_TOPO_PLAQUETTE: return o[1];
+        case GF_TOPO_CLOVER: return o[2];
+        case GF_TOPO_IMPROVED: return (5.0 / 3.0) * o[2] + (-1.0 / 12.0) * o[3];
+        default: return o[3];
+    }
+}
+
 // ---- stout helpers -------------------------------------------------------------------
 int stout_alloc(gf_ctx* c) {
     for (int l = 0; l < c->nlev; l++) {
         TRY(alloc(c, &c->lev[l], link_bytes(c)));
-        if (c->smode == GF_STOUT_STORE) TRY(alloc(c, &c->Cst[l], link_bytes(c)));
+        if (c->smode == GF_STOUT_STORE) TRY(alloc(c, &c->Cst[l], mat_bytes(c)));
     }
-    TRY(alloc(c, &c->D, link_bytes(c)));
-    TRY(alloc(c, &c->D2, link_bytes(c)));
-    TRY(alloc(c, &c->G, link_bytes(c)));
+    TRY(alloc(c, &c->D, mat_bytes(c)));
+    TRY(alloc(c, &c->D2, mat_bytes(c)));
+    TRY(alloc(c, &c->G, mat_bytes(c)));
     return GF_OK;
 }
 const void* level_in(gf_ctx* c, int l) { return l == 0 ? c->U : c->lev[l - 1]; }
@@ -294,10 +340,10 @@ int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
         if (hipEventCreate(&c->ev[i]) != hipSuccess) return bail(GF_EDEVICE);
     if (alloc(c, &c->U, link_bytes(c)) || alloc(c, &c->Ualt, link_bytes(c)) || alloc(c, &c->P, mom_bytes(c)) ||
         alloc(c, &c->F, mom_bytes(c)) ||
-        alloc(c, (void**)&c->partial, sizeof(double) * std::max(grid_for(c->V), grid_sites_mu(c->V))) ||
-        alloc(c, (void**)&c->dres, sizeof(double)))
+        alloc(c, (void**)&c->partial, sizeof(double) * 4 * std::max(grid_for(c->V), grid_sites_mu(c->V))) ||
+        alloc(c, (void**)&c->dres, 4 * sizeof(double)))
         return bail(GF_ENOMEM);
-    if (hipHostMalloc((void**)&c->hres, sizeof(double), hipHostMallocDefault) != hipSuccess) return bail(GF_ENOMEM);
+    if (hipHostMalloc((void**)&c->hres, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) return bail(GF_ENOMEM);
     int r = gf_set_cold(c);
     if (r == GF_OK) r = gf_clear_momenta(c);
     if (r == GF_OK) r = gf_sync(c);
@@ -352,25 +398,53 @@ int gf_set_cold(gf_ctx* c) {
     });
 }
 
+// The device keeps rows 0 and 1 of each link and rebuilds row 2 as conj(row0 x row1), which
+// holds for SU(3).  Links whose third row differs from that by more than kSu3Tol are rejected.
+constexpr double kSu3Tol = 1e-6;
+static double su3_row2_deviation(const double* ref, size_t nlinks) {
+    double worst = 0.0;
+    for (size_t l = 0; l < nlinks; l++) {
+        const double* m = ref + 18 * l;  // element (i, j) at complex offset i + 3 j
+        auto re = [&](int i, int j) { return m[2 * (i + 3 * j)]; };
+        auto im = [&](int i, int j) { return m[2 * (i + 3 * j) + 1]; };
+        for (int k = 0; k < 3; k++) {
+            const int a = (k + 1) % 3, b = (k + 2) % 3;
+            const double cr = (re(0, a) * re(1, b) - im(0, a) * im(1, b)) - (re(0, b) * re(1, a) - im(0, b) * im(1, a));
+            const double ci = (re(0, a) * im(1, b) + im(0, a) * re(1, b)) - (re(0, b) * im(1, a) + im(0, b) * re(1, a));
+            const double d = std::fabs(re(2, k) - cr) + std::fabs(im(2, k) + ci);
+            if (!(d <= worst)) worst = d;  // also catches NaN
+        }
+    }
+    return worst;
+}
+
 int gf_upload_links(gf_ctx* c, const double* ref) {
     CTXCHK(c);
     if (!ref) return fail(c, GF_EINVAL, "null links");
+    {
+        const double dev = su3_row2_deviation(ref, (size_t)4 * c->V);
+        if (!(dev <= kSu3Tol))
+            return fail(c, GF_EINVAL,
+                        "links must be SU(3): the third row differs from conj(row1 x row2) by %g (limit %g)", dev,
+                        kSu3Tol);
+    }
     TRY(need_staging(c));
     HIPCHK(c, hipMemcpyAsync(c->staging, ref, (size_t)72 * c->V * sizeof(double), hipMemcpyHostToDevice, c->stream));
     TRY(dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
-        HIPCHK(c, launch_links_from_ref<T>(c->staging, (T*)c->U, c->V, c->stream));
+        HIPCHK(c, launch_links_from_ref<T>(c->staging, (T*)c->U, c->V, false, c->stream));
         return GF_OK;
     }));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GF_OK;
 }
 
-static int download_linkfield(gf_ctx* c, const void* field, double* ref) {
+// general: the field holds general 3x3 matrices (derivatives) rather than SU(3) links
+static int download_linkfield(gf_ctx* c, const void* field, double* ref, bool general = false) {
     TRY(need_staging(c));
     TRY(dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
-        HIPCHK(c, launch_links_to_ref<T>((const T*)field, c->staging, c->V, c->stream));
+        HIPCHK(c, launch_links_to_ref<T>((const T*)field, c->staging, c->V, general, c->stream));
         return GF_OK;
     }));
     HIPCHK(c, hipMemcpyAsync(ref, c->staging, (size_t)72 * c->V * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -378,12 +452,12 @@ static int download_linkfield(gf_ctx* c, const void* field, double* ref) {
     return GF_OK;
 }
 
-static int upload_linkfield(gf_ctx* c, void* field, const double* ref) {
+static int upload_linkfield(gf_ctx* c, void* field, const double* ref, bool general = false) {
     TRY(need_staging(c));
     HIPCHK(c, hipMemcpyAsync(c->staging, ref, (size_t)72 * c->V * sizeof(double), hipMemcpyHostToDevice, c->stream));
     TRY(dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
-        HIPCHK(c, launch_links_from_ref<T>(c->staging, (T*)field, c->V, c->stream));
+        HIPCHK(c, launch_links_from_ref<T>(c->staging, (T*)field, c->V, general, c->stream));
         return GF_OK;
     }));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -487,13 +561,13 @@ int gf_hamiltonian(gf_ctx* c, double coeff, double* H) {
 
 int gf_dsdu(gf_ctx* c, double coeff, double* ref) {
     CTXCHK(c);
-    TRY(alloc(c, &c->D, link_bytes(c)));
+    TRY(alloc(c, &c->D, mat_bytes(c)));
     TRY(dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
         HIPCHK(c, launch_dsdu<T>(c->geo, (const T*)c->U, coeff, (T*)c->D, c->stream));
         return GF_OK;
     }));
-    if (ref) return download_linkfield(c, c->D, ref);
+    if (ref) return download_linkfield(c, c->D, ref, true);
     return GF_OK;
 }
 
@@ -586,6 +660,80 @@ int gf_energy_density(gf_ctx* c, double* E) {
     return GF_OK;
 }
 
+int gf_gaussian_momenta(gf_ctx* c, double sigma, unsigned long long seed, unsigned long long sweep) {
+    CTXCHK(c);
+    if (!(sigma >= 0.0) || !std::isfinite(sigma)) return fail(c, GF_EINVAL, "sigma must be finite and >= 0; got %g", sigma);
+    return dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_gaussian_momenta<T>(c->geo, (T*)c->P, sigma, seed, sweep, c->stream));
+        return GF_OK;
+    });
+}
+
+int gf_hot_start(gf_ctx* c, unsigned long long seed) {
+    CTXCHK(c);
+    return dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_hot_links<T>(c->geo, (T*)c->U, seed, c->stream));
+        return GF_OK;
+    });
+}
+
+int gf_topological_charge(gf_ctx* c, int method, double* Q) {
+    CTXCHK(c);
+    if (!Q) return fail(c, GF_EINVAL, "Q is required");
+    int sets;
+    TRY(topo_sets(c, method, &sets));
+    double o[4];
+    TRY(fstrength(c, sets, o, nullptr, 0));
+    *Q = topo_of(method, o);
+    return GF_OK;
+}
+
+int gf_topological_charge_density(gf_ctx* c, int method, double* density) {
+    CTXCHK(c);
+    if (!density) return fail(c, GF_EINVAL, "density is required");
+    int sets;
+    TRY(topo_sets(c, method, &sets));
+    double o[4];
+    return fstrength(c, sets, o, density, method);
+}
+
+int gf_energy_density_clover(gf_ctx* c, double* E) {
+    CTXCHK(c);
+    if (!E) return fail(c, GF_EINVAL, "E is required");
+    double o[4];
+    TRY(fstrength(c, 2, o, nullptr, 0));
+    *E = o[0] / (double)c->V;
+    return GF_OK;
+}
+
+int gf_flow_measure(gf_ctx* c, double eps, int nsteps, int every, double* out) {
+    CTXCHK(c);
+    if (every <= 0) return fail(c, GF_EINVAL, "every must be positive; got %d", every);
+    if (!out) return fail(c, GF_EINVAL, "out is required");
+    int k = 0;
+    for (int done = 0; done < nsteps;) {
+        const int n = std::min(every, nsteps - done);
+        TRY(gf_flow(c, eps, n, 0, nullptr));
+        done += n;
+        if (n == every) {
+            double plaq, o[4];
+            TRY(plaq_of(c, c->U, &plaq));
+            TRY(fstrength(c, 7, o, nullptr, 0));
+            double* r = out + (size_t)GF_FLOW_NOBS * k++;
+            r[0] = eps * done;
+            r[1] = plaq / (18.0 * (double)c->V);
+            r[2] = 2.0 * (18.0 - plaq / (double)c->V);
+            r[3] = o[0] / (double)c->V;
+            r[4] = o[1];
+            r[5] = o[2];
+            r[6] = topo_of(GF_TOPO_IMPROVED, o);
+        }
+    }
+    return GF_OK;
+}
+
 int gf_flow(gf_ctx* c, double eps, int nsteps, int every, double* E_out) {
     CTXCHK(c);
     if (nsteps <= 0) return fail(c, GF_EINVAL, "steps must be positive; got %d", nsteps);
@@ -622,14 +770,14 @@ int gf_stout_config(gf_ctx* c, int nlev, double rho, int mode) {
         size_t need_levels = 0, need_c = 0;
         for (int l = 0; l < nlev; l++) {
             if (!c->lev[l]) need_levels += link_bytes(c);
-            if (!c->Cst[l]) need_c += link_bytes(c);
+            if (!c->Cst[l]) need_c += mat_bytes(c);
         }
-        need_levels += (c->D ? 0 : link_bytes(c)) + (c->D2 ? 0 : link_bytes(c)) + (c->G ? 0 : link_bytes(c));
+        need_levels += (c->D ? 0 : mat_bytes(c)) + (c->D2 ? 0 : mat_bytes(c)) + (c->G ? 0 : mat_bytes(c));
         mode = (double)(need_levels + need_c) * 1.1 <= (double)free_b ? GF_STOUT_STORE : GF_STOUT_RECOMPUTE;
     }
     // release stored C of a previous store-mode configuration if no longer needed
     for (int l = 0; l < GF_MAX_LEV; l++)
-        if (l >= nlev || mode != GF_STOUT_STORE) release(c, &c->Cst[l], link_bytes(c));
+        if (l >= nlev || mode != GF_STOUT_STORE) release(c, &c->Cst[l], mat_bytes(c));
     for (int l = nlev; l < GF_MAX_LEV; l++) release(c, &c->lev[l], link_bytes(c));
     c->nlev = nlev;
     c->rho = rho;
@@ -656,9 +804,9 @@ int gf_stout_backward(gf_ctx* c, const double* dout_ref, double* din_ref) {
     CTXCHK(c);
     if (c->nlev == 0) return fail(c, GF_ESTATE, "no stout levels configured");
     if (!dout_ref || !din_ref) return fail(c, GF_EINVAL, "null derivative buffer");
-    TRY(upload_linkfield(c, c->D, dout_ref));
+    TRY(upload_linkfield(c, c->D, dout_ref, true));
     TRY(stout_backward_all(c));
-    return download_linkfield(c, c->D, din_ref);
+    return download_linkfield(c, c->D, din_ref, true);
 }
 
 int gf_stout_force(gf_ctx* c, double coeff) {

@@ -23,6 +23,8 @@ inline int grid_sites_mu(int V) { return (V + 63) / 64; }
 enum ForceMode { FORCE_STORE = 0, FORCE_KICK = 1, FORCE_KICK_EXP = 2 };
 
 // ---- k_gauge.hip ----
+// out[0] = sum of partial[0..n) (one workgroup, fixed order)
+hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStream_t st);
 template <typename T>
 hipError_t launch_plaquette(const Geo& g, const T* U, double* partial, double* out, hipStream_t st);
 template <typename T>
@@ -40,10 +42,12 @@ hipError_t launch_exp_update(const Geo& g, const T* Uin, T* Uout, const T* P, do
 template <typename T>
 hipError_t launch_force_from_deriv(int mode, const Geo& g, const T* U, const T* D, T* FP, double eps,
                                    hipStream_t st);
+// reference-layout (column-major 3x3 per link) <-> device field; general = false: SU(3) link
+// field (rows 0 and 1 kept, row 2 rebuilt on the way out), true: general matrix field
 template <typename T>
-hipError_t launch_links_from_ref(const double* ref, T* soa, int V, hipStream_t st);
+hipError_t launch_links_from_ref(const double* ref, T* soa, int V, bool general, hipStream_t st);
 template <typename T>
-hipError_t launch_links_to_ref(const T* soa, double* ref, int V, hipStream_t st);
+hipError_t launch_links_to_ref(const T* soa, double* ref, int V, bool general, hipStream_t st);
 template <typename T>
 hipError_t launch_momenta_from_ref(const double* ref, T* soa, int V, hipStream_t st);
 template <typename T>
@@ -54,6 +58,21 @@ template <typename T>
 hipError_t launch_fill(T* p, size_t n, T v, hipStream_t st);
 template <typename T>
 hipError_t launch_axpby(T* y, const T* x, double a, double b, size_t n, hipStream_t st);
+
+// ---- k_fstrength.hip ----
+// Field strengths of the loop sets in `sets` (bit 0 plaquette, 1 clover, 2 rectangle); out4 =
+// {clover energy sum, plaquette Q, clover Q, rectangle Q} (device doubles); dens (device, V) if
+// non-null receives the density of dens_method (0 plaquette, 1 clover, 2 improved, 3 rectangle).
+template <typename T>
+hipError_t launch_field_strength(const Geo& g, const T* U, int sets, double* partial, double* out4, double* dens,
+                                 int dens_method, hipStream_t st);
+
+// ---- k_random.hip ----
+template <typename T>
+hipError_t launch_gaussian_momenta(const Geo& g, T* P, double sigma, unsigned long long seed,
+                                   unsigned long long sweep, hipStream_t st);
+template <typename T>
+hipError_t launch_hot_links(const Geo& g, T* U, unsigned long long seed, hipStream_t st);
 
 // ---- k_flow.hip ----
 // A = alpha * TA(W V^dag) + beta * A ; Wout = exp(A) W

@@ -20,7 +20,7 @@ __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restric
     decode(g, s, x);
     const unsigned V = g.V;
     const M3<T> sig = staple_sigma(g, W, V, s, x, mu);
-    const M3<T> w = ldl(W, V, mu, s);
+    const M3<T> w = ldu(W, V, mu, s);
     T c[8];
     ta_coeffs(mul(w, sig), c);
     if (beta != T(0)) {
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restric
     stp<T, GF_ST_NT>(A, V, mu, s, c);
     M3<T> wn = mul(expi_herm(herm_from_coeffs(c, T(1))), w);
     if (kFloatBulk<T>) reunitarize(wn);
-    stl<T, GF_ST_NT>(Wout, V, mu, s, wn);
+    stu<T, GF_ST_NT>(Wout, V, mu, s, wn);
 }
 
 // ---------------------------------------------------------------------------
@@ -53,10 +53,10 @@ __global__ __launch_bounds__(kBlock) void k_stout_forward(Geo g, const T* __rest
     decode(g, s, x);
     const unsigned V = g.V;
     const M3<T> C = scal(rho, dag(staple_sigma(g, Uin, V, s, x, mu)));
-    const M3<T> u = ldl(Uin, V, mu, s);
+    const M3<T> u = ldu(Uin, V, mu, s);
     const M3<T> E = exp_antiherm(ta_mat(mul_nd(C, u)));
-    stl<T, GF_ST_NT>(Uout, V, mu, s, mul(E, u));
-    if (STORE) stl<T, GF_ST_NT>(Cst, V, mu, s, C);
+    stu<T, GF_ST_NT>(Uout, V, mu, s, mul(E, u));
+    if (STORE) stm<T, GF_ST_NT>(Cst, V, mu, s, C);
 }
 
 // ---------------------------------------------------------------------------
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
     const unsigned V = g.V;
     M3<T> C;
     if (STORED) {
-        C = ldl(Cst, V, mu, s);
+        C = ldm(Cst, V, mu, s);
     } else {
         int x[4];
         decode(g, s, x);
@@ -83,17 +83,17 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
     }
     M3<T> E, L;
     {
-        const M3<T> u = ldl(Uin, V, mu, s);
-        exp_dexp(ta_mat(mul_nd(C, u)), mul(u, ldl(Dout, V, mu, s)), E, L);
+        const M3<T> u = ldu(Uin, V, mu, s);
+        exp_dexp(ta_mat(mul_nd(C, u)), mul(u, ldm(Dout, V, mu, s)), E, L);
     }
     const M3<T> Lam = ta_mat(L);
     // re-read U_mu and Dout_mu (L1/L2 hits) instead of keeping them live across exp_dexp
     int s1 = s;
     asm volatile("" : "+v"(s1));
-    stl<T, GF_ST_NT>(G, V, mu, s, mul_dn(ldl(Uin, V, mu, s1), Lam));
-    M3<T> D = mul(ldl(Dout, V, mu, s1), E);
+    stm<T, GF_ST_NT>(G, V, mu, s, mul_dn(ldu(Uin, V, mu, s1), Lam));
+    M3<T> D = mul(ldm(Dout, V, mu, s1), E);
     acc_dag(D, mul(Lam, C));
-    stl<T, GF_ST_NT>(Din, V, mu, s, D);
+    stm<T, GF_ST_NT>(Din, V, mu, s, D);
 }
 
 // ---------------------------------------------------------------------------
@@ -127,35 +127,35 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* _
             {  // U_b(y+a) [G_a(y+b) U_b(y)^dag + U_a(y+b)^dag G_b(y)]
                 int y0 = y, y1 = ypa, ypb = ypb0;
                 asm volatile("" : "+v"(y0), "+v"(y1), "+v"(ypb));
-                M3<T> t = mul_nd(ldl(G, V, a, ypb), ldl(U, V, b, y0));
-                acc_mul_dn(t, ldl(U, V, a, ypb), ldl(G, V, b, y0));
-                acc_mul(acc, ldl(U, V, b, y1), t);
+                M3<T> t = mul_nd(ldm(G, V, a, ypb), ldu(U, V, b, y0));
+                acc_mul_dn(t, ldu(U, V, a, ypb), ldm(G, V, b, y0));
+                acc_mul(acc, ldu(U, V, b, y1), t);
             }
             {  // U_b(y-b+a)^dag [G_a(y-b) U_b(y-b) + (G_b(y-b) U_a(y-b))^dag]
                 int ymb = ymb0, ypamb = ypamb0;
                 asm volatile("" : "+v"(ymb), "+v"(ypamb));
-                M3<T> t = mul(ldl(G, V, a, ymb), ldl(U, V, b, ymb));
-                acc_dag(t, mul(ldl(G, V, b, ymb), ldl(U, V, a, ymb)));
-                acc_mul_dn(acc, ldl(U, V, b, ypamb), t);
+                M3<T> t = mul(ldm(G, V, a, ymb), ldu(U, V, b, ymb));
+                acc_dag(t, mul(ldm(G, V, b, ymb), ldu(U, V, a, ymb)));
+                acc_mul_dn(acc, ldu(U, V, b, ypamb), t);
             }
             {  // (U_b(y) U_a(y+b) G_b(y+a))^dag
                 int y0 = y, y1 = ypa, ypb = ypb0;
                 asm volatile("" : "+v"(y0), "+v"(y1), "+v"(ypb));
-                acc_dag(acc, mul(mul(ldl(U, V, b, y0), ldl(U, V, a, ypb)), ldl(G, V, b, y1)));
+                acc_dag(acc, mul(mul(ldu(U, V, b, y0), ldu(U, V, a, ypb)), ldm(G, V, b, y1)));
             }
             {  // G_b(y+a-b) U_a(y-b)^dag U_b(y-b)
                 int ymb = ymb0, ypamb = ypamb0;
                 asm volatile("" : "+v"(ymb), "+v"(ypamb));
-                acc_mul(acc, ldl(G, V, b, ypamb), mul_dn(ldl(U, V, a, ymb), ldl(U, V, b, ymb)));
+                acc_mul(acc, ldm(G, V, b, ypamb), mul_dn(ldu(U, V, a, ymb), ldu(U, V, b, ymb)));
             }
         }
-        M3<T> d = ldl(Din, V, a, y);
+        M3<T> d = ldm(Din, V, a, y);
 #pragma unroll
         for (int k = 0; k < 9; k++) {
             d.m[k].r = fma(rho, acc.m[k].r, d.m[k].r);
             d.m[k].i = fma(rho, acc.m[k].i, d.m[k].i);
         }
-        stl<T, GF_ST_NT>(Din, V, a, y, d);
+        stm<T, GF_ST_NT>(Din, V, a, y, d);
     }
 }
 

@@ -28,8 +28,8 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(Geo g, const T* __restrict
             const int nu = pl < 3 ? pl + 1 : (pl < 5 ? pl - 1 : 3);
             const int spm = s + dlt(g, x, mu, 1), spn = s + dlt(g, x, nu, 1);
             // products in FP64 also for FP32 fields: the action/Hamiltonian sums carry no float rounding
-            const M3<double> a = mul(to_f64(ldl(U, V, mu, s)), to_f64(ldl(U, V, nu, spm)));  // U_mu(x) U_nu(x+mu)
-            const M3<double> b = mul(to_f64(ldl(U, V, nu, s)), to_f64(ldl(U, V, mu, spn)));  // U_nu(x) U_mu(x+nu)
+            const M3<double> a = mul(to_f64(ldu(U, V, mu, s)), to_f64(ldu(U, V, nu, spm)));  // U_mu(x) U_nu(x+mu)
+            const M3<double> b = mul(to_f64(ldu(U, V, nu, s)), to_f64(ldu(U, V, mu, spn)));  // U_nu(x) U_mu(x+nu)
             v += retr_nd(a, b);
         }
     }
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U,
     if (sm.s >= g.V) return;
     int x[4];
     decode(g, sm.s, x);
-    stl(D, (unsigned)g.V, sm.mu, (unsigned)sm.s, scal(coeff, staple_sigma(g, U, (unsigned)g.V, sm.s, x, sm.mu)));
+    stm(D, (unsigned)g.V, sm.mu, (unsigned)sm.s, scal(coeff, staple_sigma(g, U, (unsigned)g.V, sm.s, x, sm.mu)));
 }
 
 // K3 (+K4): F_mu = -(coeff/3) TA(U_mu Sigma_mu)   (md_force!, molecular_dynamics.jl:251-267)
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U
     decode(g, s, x);
     const unsigned V = g.V;
     const M3<T> sig = staple_sigma(g, U, V, s, x, mu);
-    const M3<T> umu = ldl(U, V, mu, s);
+    const M3<T> umu = ldu(U, V, mu, s);
     T c[8];
     ta_coeffs(mul(umu, sig), c);
     if (MODE == FORCE_STORE) {
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U
         if (MODE == FORCE_KICK_EXP) {
             M3<T> un = mul(expi_herm(herm_from_coeffs(p, eps_u)), umu);
             if (kFloatBulk<T>) reunitarize(un);
-            stl<T, GF_ST_NT>(Unew, V, mu, s, un);
+            stu<T, GF_ST_NT>(Unew, V, mu, s, un);
         }
     }
 }
@@ -122,9 +122,9 @@ __global__ __launch_bounds__(kBlock) void k_exp_update(Geo g, const T* __restric
     for (int mu = 0; mu < 4; mu++) {
         T p[8];
         ldp(P, V, mu, s, p);
-        M3<T> un = mul(expi_herm(herm_from_coeffs(p, eps)), ldl(Uin, V, mu, s));
+        M3<T> un = mul(expi_herm(herm_from_coeffs(p, eps)), ldu(Uin, V, mu, s));
         if (kFloatBulk<T>) reunitarize(un);
-        stl<T, GF_ST_NT>(Uout, V, mu, s, un);
+        stu<T, GF_ST_NT>(Uout, V, mu, s, un);
     }
 }
 
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_force_from_deriv(Geo g, const T* __r
     const unsigned V = g.V;
     const T fac = T(-1.0 / 3.0);
     T c[8];
-    ta_coeffs(mul(ldl(U, V, mu, s), ldl(D, V, mu, s)), c);
+    ta_coeffs(mul(ldu(U, V, mu, s), ldm(D, V, mu, s)), c);
     if (MODE == FORCE_STORE) {
 #pragma unroll
         for (int a = 0; a < 8; a++) c[a] *= fac;
@@ -158,7 +158,8 @@ __global__ __launch_bounds__(kBlock) void k_force_from_deriv(Geo g, const T* __r
 // layout conversion between the reference's (3,3,X,Y,Z,T) complex column-major arrays
 // (FP64, per direction) and the SoA planes
 // ---------------------------------------------------------------------------
-template <typename T>
+// GEN = false: SU(3) link field (rows 0 and 1 stored); GEN = true: general 3x3 matrix field
+template <typename T, bool GEN>
 __global__ __launch_bounds__(kBlock) void k_links_from_ref(const double* __restrict__ ref, T* __restrict__ soa, int V) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= V) return;
@@ -173,16 +174,19 @@ __global__ __launch_bounds__(kBlock) void k_links_from_ref(const double* __restr
                 m.m[3 * i + j].r = (T)r[2 * c];
                 m.m[3 * i + j].i = (T)r[2 * c + 1];
             }
-        stl(soa, (unsigned)V, mu, (unsigned)s, m);
+        if (GEN)
+            stm(soa, (unsigned)V, mu, (unsigned)s, m);
+        else
+            stu(soa, (unsigned)V, mu, (unsigned)s, m);
     }
 }
-template <typename T>
+template <typename T, bool GEN>
 __global__ __launch_bounds__(kBlock) void k_links_to_ref(const T* __restrict__ soa, double* __restrict__ ref, int V) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= V) return;
     for (int mu = 0; mu < 4; mu++) {
         double* r = ref + ((size_t)mu * V + s) * 18;
-        const M3<T> m = ldl(soa, (unsigned)V, mu, (unsigned)s);
+        const M3<T> m = GEN ? ldm(soa, (unsigned)V, mu, (unsigned)s) : ldu(soa, (unsigned)V, mu, (unsigned)s);
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -219,7 +223,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_set_identity(T* __restrict__ U, int V) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= V) return;
-    for (int mu = 0; mu < 4; mu++) stl(U, (unsigned)V, mu, (unsigned)s, ident3<T>());
+    for (int mu = 0; mu < 4; mu++) stu(U, (unsigned)V, mu, (unsigned)s, ident3<T>());
 }
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill(T* __restrict__ p, size_t n, T v) {
@@ -235,6 +239,11 @@ __global__ __launch_bounds__(kBlock) void k_axpby(T* __restrict__ y, const T* __
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStream_t st) {
+    k_sum_partials<<<1, 1024, 0, st>>>(partial, n, out);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_plaquette(const Geo& g, const T* U, double* partial, double* out, hipStream_t st) {
     const int nb = grid_sites_mu(g.V);
@@ -291,13 +300,19 @@ hipError_t launch_force_from_deriv(int mode, const Geo& g, const T* U, const T* 
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_links_from_ref(const double* ref, T* soa, int V, hipStream_t st) {
-    k_links_from_ref<T><<<grid_for(V), kBlock, 0, st>>>(ref, soa, V);
+hipError_t launch_links_from_ref(const double* ref, T* soa, int V, bool general, hipStream_t st) {
+    if (general)
+        k_links_from_ref<T, true><<<grid_for(V), kBlock, 0, st>>>(ref, soa, V);
+    else
+        k_links_from_ref<T, false><<<grid_for(V), kBlock, 0, st>>>(ref, soa, V);
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_links_to_ref(const T* soa, double* ref, int V, hipStream_t st) {
-    k_links_to_ref<T><<<grid_for(V), kBlock, 0, st>>>(soa, ref, V);
+hipError_t launch_links_to_ref(const T* soa, double* ref, int V, bool general, hipStream_t st) {
+    if (general)
+        k_links_to_ref<T, true><<<grid_for(V), kBlock, 0, st>>>(soa, ref, V);
+    else
+        k_links_to_ref<T, false><<<grid_for(V), kBlock, 0, st>>>(soa, ref, V);
     return hipGetLastError();
 }
 template <typename T>
@@ -337,8 +352,8 @@ hipError_t launch_axpby(T* y, const T* x, double a, double b, size_t n, hipStrea
     template hipError_t launch_exp_update<T>(const Geo&, const T*, T*, const T*, double, hipStream_t);   \
     template hipError_t launch_force_from_deriv<T>(int, const Geo&, const T*, const T*, T*, double,      \
                                                    hipStream_t);                                         \
-    template hipError_t launch_links_from_ref<T>(const double*, T*, int, hipStream_t);                   \
-    template hipError_t launch_links_to_ref<T>(const T*, double*, int, hipStream_t);                     \
+    template hipError_t launch_links_from_ref<T>(const double*, T*, int, bool, hipStream_t);             \
+    template hipError_t launch_links_to_ref<T>(const T*, double*, int, bool, hipStream_t);               \
     template hipError_t launch_momenta_from_ref<T>(const double*, T*, int, hipStream_t);                 \
     template hipError_t launch_momenta_to_ref<T>(const T*, double*, int, hipStream_t);                   \
     template hipError_t launch_set_identity<T>(T*, int, hipStream_t);                                    \

@@ -1,8 +1,12 @@
 // lattice.hpp -- site geometry and structure-of-arrays field access for the gfx950 kernels.
 //
 // HBM layout (one allocation per field, T = double or float, T2 = a (T, T) pair):
-//   link field     : 4 directions x 9 complex planes x V sites of T2; plane (mu*9 + 3*i + j)
-//                    holds element (i, j) of U_mu as (re, im)
+//   link field     : 4 directions x 6 complex planes x V sites of T2; plane (mu*6 + 3*i + j)
+//                    holds element (i, j), i = 0, 1, of the SU(3) link U_mu as (re, im).  The
+//                    third row is not stored: loads rebuild it as conj(row0 x row1), the
+//                    unitarity + det 1 identity of SU(3) (12 of 18 reals move through HBM)
+//   matrix field   : 4 directions x 9 complex planes (general 3x3 complex: derivatives,
+//                    staple sums); plane (mu*9 + 3*i + j) holds element (i, j)
 //   momentum field : 4 directions x 4 pair planes x V sites of T2; plane (mu*4 + j) holds the
 //                    coefficients (2j, 2j+1) of i lambda_a/2
 // Consecutive sites x, x+1, ... are consecutive in every plane, so one wavefront of 64
@@ -133,9 +137,45 @@ __device__ __forceinline__ void buf_st(float2 v, __amdgpu_buffer_rsrc_t r, unsig
                                           (int)voff, (int)soff, AUX);
 }
 
-// link field: plane (mu*9 + k) holds complex element k = 3*i + j of U_mu for all sites
+// SU(3) link field: planes (mu*6 + k), k = 3*i + j < 6, hold rows 0 and 1 of U_mu
 template <typename T>
-__device__ __forceinline__ M3<T> ldl(const T* __restrict__ p, unsigned V, int mu, unsigned s) {
+__device__ __forceinline__ M3<T> ldu(const T* __restrict__ p, unsigned V, int mu, unsigned s) {
+    const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 6u * mu, 6u);
+    const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
+    M3<T> m;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const T2<T> v = buf_ld(r, off, k * ps, (const T*)nullptr);
+        m.m[k].r = v.x;
+        m.m[k].i = v.y;
+    }
+    // row2_k = conj(row0_{k+1} row1_{k+2} - row0_{k+2} row1_{k+1})
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int a = (k + 1) % 3, b = (k + 2) % 3;
+        const cx<T> p0 = m.m[a], p1 = m.m[b], q0 = m.m[3 + a], q1 = m.m[3 + b];
+        m.m[6 + k].r = (p0.r * q1.r - p0.i * q1.i) - (p1.r * q0.r - p1.i * q0.i);
+        m.m[6 + k].i = -((p0.r * q1.i + p0.i * q1.r) - (p1.r * q0.i + p1.i * q0.r));
+    }
+    return m;
+}
+
+template <typename T, int AUX = GF_ST_DEFAULT>
+__device__ __forceinline__ void stu(T* __restrict__ p, unsigned V, int mu, unsigned s, const M3<T>& m) {
+    const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 6u * mu, 6u);
+    const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        T2<T> v;
+        v.x = m.m[k].r;
+        v.y = m.m[k].i;
+        buf_st<AUX>(v, r, off, k * ps);
+    }
+}
+
+// general matrix field: plane (mu*9 + k) holds complex element k = 3*i + j for all sites
+template <typename T>
+__device__ __forceinline__ M3<T> ldm(const T* __restrict__ p, unsigned V, int mu, unsigned s) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 9u * mu, 9u);
     const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
     M3<T> m;
@@ -149,7 +189,7 @@ __device__ __forceinline__ M3<T> ldl(const T* __restrict__ p, unsigned V, int mu
 }
 
 template <typename T, int AUX = GF_ST_DEFAULT>
-__device__ __forceinline__ void stl(T* __restrict__ p, unsigned V, int mu, unsigned s, const M3<T>& m) {
+__device__ __forceinline__ void stm(T* __restrict__ p, unsigned V, int mu, unsigned s, const M3<T>& m) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 9u * mu, 9u);
     const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
 #pragma unroll
@@ -209,15 +249,15 @@ __device__ __forceinline__ M3<T> staple_sigma(const Geo& g, const T* __restrict_
         {  // upper: (U_nu(x+mu) U_mu(x+nu)^dag) U_nu(x)^dag
             int s0 = s, s1 = spm, spn = s + dlt(g, x, nu, 1);
             asm volatile("" : "+v"(s0), "+v"(s1), "+v"(spn));
-            const M3<T> a = mul_nd(ldl(U, V, nu, s1), ldl(U, V, mu, spn));
-            acc_mul_nd(sig, a, ldl(U, V, nu, s0));
+            const M3<T> a = mul_nd(ldu(U, V, nu, s1), ldu(U, V, mu, spn));
+            acc_mul_nd(sig, a, ldu(U, V, nu, s0));
         }
         {  // lower: (U_mu(x-nu) U_nu(x+mu-nu))^dag U_nu(x-nu)
             const int dmn = dlt(g, x, nu, -1);
             int smn = s + dmn, spmmn = spm + dmn;
             asm volatile("" : "+v"(smn), "+v"(spmmn));
-            const M3<T> b = mul(ldl(U, V, mu, smn), ldl(U, V, nu, spmmn));
-            acc_mul_dn(sig, b, ldl(U, V, nu, smn));
+            const M3<T> b = mul(ldu(U, V, mu, smn), ldu(U, V, nu, spmmn));
+            acc_mul_dn(sig, b, ldu(U, V, nu, smn));
         }
     }
     return sig;

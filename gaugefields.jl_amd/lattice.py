@@ -131,6 +131,15 @@ class Lattice:
 
         write_ildg(path, self.dims, self.download_links(), precision)
 
+    def gaussian_momenta(self, seed, sweep=0, sigma=1.0):
+        """P_a ~ N(0, sigma^2) generated on the device (Philox4x32-10 stream (seed, sweep))"""
+        self._c(self._L.gf_gaussian_momenta(self._ctx, ctypes.c_double(sigma), ctypes.c_ulonglong(seed),
+                                            ctypes.c_ulonglong(sweep)))
+
+    def hot_start(self, seed):
+        """random SU(3) links generated on the device (Philox4x32-10 stream seed)"""
+        self._c(self._L.gf_hot_start(self._ctx, ctypes.c_ulonglong(seed)))
+
     def links_from_algebra(self, omega):
         """U = exp(i sum_a omega_a lambda_a / 2) per link (the configs' seeded start)."""
         self.set_cold()
@@ -161,6 +170,36 @@ class Lattice:
 
     def energy_density(self):
         return self._scalar(self._L.gf_energy_density)
+
+    # -- field strengths --------------------------------------------------------------
+    TOPO_METHODS = {"plaquette": _lib.GF_TOPO_PLAQUETTE, "clover": _lib.GF_TOPO_CLOVER,
+                    "improved": _lib.GF_TOPO_IMPROVED, "rectangle": _lib.GF_TOPO_RECTANGLE}
+
+    def _topo(self, method):
+        if method not in self.TOPO_METHODS:
+            raise ValueError(f"method must be one of {list(self.TOPO_METHODS)}; got {method!r}")
+        return self.TOPO_METHODS[method]
+
+    def topological_charge(self, method="plaquette"):
+        out = ctypes.c_double()
+        self._c(self._L.gf_topological_charge(self._ctx, self._topo(method), ctypes.byref(out)))
+        return out.value
+
+    def topological_charge_density(self, method="plaquette"):
+        """per-site density in site order x + NX*(y + NY*(z + NZ*t))"""
+        q = np.empty(self.V)
+        self._c(self._L.gf_topological_charge_density(self._ctx, self._topo(method), _dptr(q)))
+        return q
+
+    def energy_density_clover(self):
+        return self._scalar(self._L.gf_energy_density_clover)
+
+    def flow_measure(self, eps, nsteps, every):
+        """rows {t, plaquette, E_plaq, E_clover, Q_plaq, Q_clover, Q_improved} every `every` steps"""
+        n = nsteps // every if every > 0 else 0
+        out = np.zeros((max(n, 1), _lib.GF_FLOW_NOBS))
+        self._c(self._L.gf_flow_measure(self._ctx, ctypes.c_double(eps), int(nsteps), int(every), _dptr(out)))
+        return out[:n]
 
     # -- derivative, force, updates -----------------------------------------------------
     def dsdu(self, coeff, download=True):

@@ -68,6 +68,18 @@ int gf_download_force(gf_ctx *ctx, double *force);
 int gf_save_links(gf_ctx *ctx);
 int gf_restore_links(gf_ctx *ctx);
 
+/* ---- random fields on the device (no counterpart layout in the reference: the generator
+ * is Philox4x32-10, Salmon et al. SC'11, one counter per link, so results do not depend on
+ * the launch shape; key = seed, counter = (site, tag << 24 | mu << 16 | block, sweep)) ---- */
+/* gaussian_momenta!(p; sigma) (API.jl:341-392, gauss_distribution!
+ * TA_gaugefields_4D_MPILattice.jl:157-193): every p_a ~ N(0, sigma^2) (Box-Muller), stream
+ * (seed, sweep) */
+int gf_gaussian_momenta(gf_ctx *ctx, double sigma, unsigned long long seed, unsigned long long sweep);
+/* gauge_configuration(...; start=:hot) (API.jl:178-251; hot fill gaugefields_4D_wing.jl:295-322
+ * + normalize3!, AbstractGaugefields.jl:3378-3457): entries (u - 1/2) + i (u' - 1/2), rows
+ * Gram-Schmidt normalized, third row conj(row1 x row2) */
+int gf_hot_start(gf_ctx *ctx, unsigned long long seed);
+
 /* ---- measurements ---------------------------------------------------------------- */
 /* calculate_Plaquette (AbstractGaugefields.jl:2684-2699): sum_{x,mu<nu} Re tr P_munu.
  * measure_plaquette (API.jl:395-404) divides by 6*V*3. */
@@ -108,6 +120,27 @@ int gf_flow(gf_ctx *ctx, double eps, int nsteps, int every, double *E_out);
 /* plaquette energy density E = 2 sum_{mu<nu} Re tr(1 - P_munu) / V = 36 (1 - plaquette)
  * (Luscher JHEP 08 (2010) 071; no built-in counterpart in the reference) */
 int gf_energy_density(gf_ctx *ctx, double *E);
+
+/* ---- field strengths: topological charge, clover energy density ------------------- */
+enum { GF_TOPO_PLAQUETTE = 0, GF_TOPO_CLOVER = 1, GF_TOPO_IMPROVED = 2, GF_TOPO_RECTANGLE = 3 };
+/* topological_charge(U; method) (AbstractGaugefields.jl:1452-1471): Q = sum_x q(x) with
+ * q = -norm Re sum eps_{mu nu rho sigma} tr(F_munu F_rhosigma) / (32 pi^2 n^2), F_munu = TA of
+ * the sum of the method's n loops at x (plaquette n = 1; clover leaves n = 4; 2x1 rectangles
+ * n = 8, norm 2); improved = 5/3 clover - 1/12 rectangle (:1195-1397).  GF_TOPO_RECTANGLE is
+ * the reference's internal _rectangle_topological_charge. */
+int gf_topological_charge(gf_ctx *ctx, int method, double *Q);
+/* topological_charge_density(U; method) (AbstractGaugefields.jl:1430-1446): density[V] on the
+ * host, site order x + NX (y + NY (z + NZ t)) */
+int gf_topological_charge_density(gf_ctx *ctx, int method, double *density);
+/* calculate_energy_density (samples/measurements/energydensity.jl:32-76): clover E =
+ * sum_{mu != nu} -tr(F_munu F_munu)/2 / (16 V) with F = TA(sum of the four clover leaves) */
+int gf_energy_density_clover(gf_ctx *ctx, double *E);
+/* flow!(U, g) steps with the measurements of samples/measurements/topologicalcharge.jl:236-245
+ * and energydensity.jl:100-118: after every `every` steps (nsteps / every rows) out receives
+ * GF_FLOW_NOBS doubles {t, plaquette / (18 V), E plaquette (as gf_energy_density), E clover,
+ * Q plaquette, Q clover, Q improved}. */
+enum { GF_FLOW_NOBS = 7 };
+int gf_flow_measure(gf_ctx *ctx, double eps, int nsteps, int every, double *out);
 
 /* ---- stout smearing ---------------------------------------------------------------- */
 /* nlev stout layers at rho (STOUT_Layer + CovNeuralnet, stout_fast.jl:6-122,

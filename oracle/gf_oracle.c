@@ -190,6 +190,7 @@ static double lattice_sum(const lat *L, site_fn f, const void *ctx) {
 /* TA projection and su(3) basis                                              */
 /* ------------------------------------------------------------------------- */
 static const double SR3I = 0.57735026918962576451; /* 1/sqrt(3) */
+static const double GFO_PI = 3.14159265358979323846;
 
 /* Traceless_antihermitian_add! for NC=3 (TA_gaugefields_4D_serial.jl:181-269):
  * c_a += factor * coefficient of TA(v) in the basis i lambda_a / 2. */
@@ -837,4 +838,256 @@ void gfo_md_trajectory(const int *dims, double *U, double *P, double coeff, int 
     H[2] = h1 - h0;
     free(m.F);
     lat_free(&L);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Field strengths: topological charge and clover energy density              */
+/* ------------------------------------------------------------------------- */
+/* Product of links along a Wilson line from site s (evaluate_gaugelinks!,
+ * AbstractGaugefields.jl:1797-1860, on Wilsonline paths): a step (mu, +n) multiplies
+ * U_mu(y) and moves y -> y + mu, n times; a step (mu, -n) moves y -> y - mu and multiplies
+ * U_mu(y)^dag, n times. */
+typedef struct {
+    int mu, len;
+} step;
+static m3 path_product(const lat *L, const double *U, long s, const step *p, int n) {
+    m3 r = m3_id();
+    long y = s;
+    for (int k = 0; k < n; k++) {
+        int mu = p[k].mu, len = p[k].len;
+        for (int q = 0; q < abs(len); q++) {
+            if (len > 0) {
+                r = mul(r, LINK(U, L, mu, y));
+                y = UP(L, mu, y);
+            } else {
+                y = DN(L, mu, y);
+                r = mul_nd(r, LINK(U, L, mu, y));
+            }
+        }
+    }
+    return r;
+}
+
+/* Loop sets per (mu, nu): plaquette (AbstractGaugefields.jl:1208), the four clover leaves
+ * (make_cloverloops, autostaples/wilsonloops.jl:166-177), the eight 2x1 rectangles
+ * (_rectangle_loops, AbstractGaugefields.jl:1314-1330). */
+static int loops_of(int method, int mu, int nu, step out[8][4]) {
+    const int m = mu, n = nu;
+    if (method == GFO_TOPO_PLAQUETTE) {
+        step a[4] = {{m, 1}, {n, 1}, {m, -1}, {n, -1}};
+        memcpy(out[0], a, sizeof a);
+        return 1;
+    }
+    if (method == GFO_TOPO_CLOVER) {
+        step l[4][4] = {{{m, 1}, {n, 1}, {m, -1}, {n, -1}},
+                        {{n, 1}, {m, -1}, {n, -1}, {m, 1}},
+                        {{n, -1}, {m, 1}, {n, 1}, {m, -1}},
+                        {{m, -1}, {n, -1}, {m, 1}, {n, 1}}};
+        memcpy(out, l, sizeof l);
+        return 4;
+    }
+    step r[8][4] = {{{m, 2}, {n, 1}, {m, -2}, {n, -1}}, {{n, 1}, {m, -2}, {n, -1}, {m, 2}},
+                    {{n, -1}, {m, 2}, {n, 1}, {m, -2}}, {{m, -2}, {n, -1}, {m, 2}, {n, 1}},
+                    {{m, 1}, {n, 2}, {m, -1}, {n, -2}}, {{n, 2}, {m, -1}, {n, -2}, {m, 1}},
+                    {{n, -2}, {m, 1}, {n, 2}, {m, -1}}, {{m, -1}, {n, -2}, {m, 1}, {n, 2}}};
+    memcpy(out, r, sizeof r);
+    return 8;
+}
+
+/* F_munu(x) = TA(sum of the method's loops at x)
+ * (_plaquette/_clover/_rectangle_field_strengths, AbstractGaugefields.jl:1195-1215,
+ * 1255-1275, 1332-1351) */
+static m3 field_strength(const lat *L, const double *U, int method, int mu, int nu, long s) {
+    step loops[8][4];
+    int nl = loops_of(method, mu, nu, loops);
+    m3 acc = m3_zero();
+    for (int k = 0; k < nl; k++) acc = add(acc, path_product(L, U, s, loops[k], 4));
+    return ta_mat(acc);
+}
+
+/* Levi-Civita symbol (_epsilon_tensor_4d, AbstractGaugefields.jl:1172-1183) */
+static int eps4(int a, int b, int c, int d) {
+    int p[4] = {a, b, c, d};
+    for (int i = 0; i < 4; i++)
+        for (int j = i + 1; j < 4; j++)
+            if (p[i] == p[j]) return 0;
+    int inv = 0;
+    for (int i = 0; i < 4; i++)
+        for (int j = i + 1; j < 4; j++) inv += p[i] > p[j];
+    return (inv % 2) ? -1 : 1;
+}
+
+/* q(x) = -norm * Re sum_{mu nu rho sigma} eps tr(F_munu F_rhosigma) / (32 pi^2 n_loops^2),
+ * norm 1 (plaquette, clover) or 2 (rectangle) (_plaquette/_clover/_rectangle_topological_
+ * charge_density, AbstractGaugefields.jl:1217-1245, 1277-1306, 1353-1385) */
+static void topo_density_basic(const lat *L, const double *U, int method, double *q) {
+    const int nl = method == GFO_TOPO_PLAQUETTE ? 1 : (method == GFO_TOPO_CLOVER ? 4 : 8);
+    const double norm = method == GFO_TOPO_RECTANGLE ? 2.0 : 1.0;
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < L->V; s++) {
+        m3 F[4][4];
+        for (int mu = 0; mu < 4; mu++)
+            for (int nu = 0; nu < 4; nu++)
+                if (mu != nu) F[mu][nu] = field_strength(L, U, method, mu, nu, s);
+        cplx acc = 0.0;
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++)
+                for (int c = 0; c < 4; c++)
+                    for (int d = 0; d < 4; d++) {
+                        int e = eps4(a, b, c, d);
+                        if (e) acc += e * trace(mul(F[a][b], F[c][d]));
+                    }
+        q[s] = -norm * creal(acc) / (32.0 * GFO_PI * GFO_PI * nl * nl);
+    }
+}
+
+/* topological_charge_density(U; method) (AbstractGaugefields.jl:1430-1446): plaquette,
+ * clover, improved = 5/3 clover - 1/12 rectangle (_improved_topological_charge_density,
+ * :1393-1397); also the rectangle density itself.  Returns the sum Q. */
+double gfo_topological_charge_density(const int *dims, const double *U, int method, double *q) {
+    lat L = lat_make(dims);
+    double *buf = q ? q : (double *)malloc(sizeof(double) * L.V);
+    if (method == GFO_TOPO_IMPROVED) {
+        double *r = (double *)malloc(sizeof(double) * L.V);
+        topo_density_basic(&L, U, GFO_TOPO_CLOVER, buf);
+        topo_density_basic(&L, U, GFO_TOPO_RECTANGLE, r);
+        for (long s = 0; s < L.V; s++) buf[s] = (5.0 / 3.0) * buf[s] + (-1.0 / 12.0) * r[s];
+        free(r);
+    } else {
+        topo_density_basic(&L, U, method, buf);
+    }
+    double Q = 0.0;
+    for (long s = 0; s < L.V; s++) Q += buf[s];
+    if (!q) free(buf);
+    lat_free(&L);
+    return Q;
+}
+
+/* clover energy density (calculate_energy_density, samples/measurements/energydensity.jl:
+ * 32-76): sum_{mu != nu} -Re tr(F_munu F_munu)/2 / (V * 4^2), F = TA(clover) */
+static double clover_energy_site(const lat *L, long s, const void *ctx) {
+    const double *U = (const double *)ctx;
+    double w = 0.0;
+    for (int mu = 0; mu < 4; mu++)
+        for (int nu = 0; nu < 4; nu++)
+            if (mu != nu) {
+                m3 F = field_strength(L, U, GFO_TOPO_CLOVER, mu, nu, s);
+                w += -creal(trace(mul(F, F))) / 2.0;
+            }
+    return w;
+}
+double gfo_energy_density_clover(const int *dims, const double *U) {
+    lat L = lat_make(dims);
+    double w = lattice_sum(&L, clover_energy_site, U);
+    double E = w / ((double)L.V * 16.0);
+    lat_free(&L);
+    return E;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Counter-based random fields (Philox4x32-10)                                */
+/* ------------------------------------------------------------------------- */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11 "Parallel random numbers: as easy as
+ * 1, 2, 3"): ten rounds of two 32x32->64 multiplies (0xD2511F53, 0xCD9E8D57) with the Weyl
+ * key schedule (0x9E3779B9, 0xBB67AE85). */
+void gfo_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+        c0 = n0;
+        c1 = n1;
+        c2 = n2;
+        c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* Stream of one link: key = seed, counter = (site, tag << 24 | mu << 16 | block, sweep).
+ * Gaussian momenta (tag 1): block b = 0..3 gives the Box-Muller pair (p_2b, p_2b+1) from two
+ * 53-bit uniforms; hot-start links (tag 2): blocks 0..4 give 18 uniforms in (0, 1). */
+static void link_block(uint64_t seed, uint64_t sweep, long s, int mu, int tag, int block, uint32_t w[4]) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    const uint32_t ctr[4] = {(uint32_t)s, ((uint32_t)tag << 24) | ((uint32_t)mu << 16) | (uint32_t)block,
+                             (uint32_t)sweep, (uint32_t)(sweep >> 32)};
+    gfo_philox4x32_10(ctr, key, w);
+}
+static inline double unit53(uint32_t hi, uint32_t lo) {
+    return ((double)(hi >> 5) * 67108864.0 + (double)(lo >> 6) + 0.5) * (1.0 / 9007199254740992.0);
+}
+static inline double unit32(uint32_t w) { return ((double)w + 0.5) * (1.0 / 4294967296.0); }
+
+/* gaussian_momenta! (API.jl:341-392; gauss_distribution!, TA_gaugefields_4D_MPILattice.jl:
+ * 157-193): p_a ~ N(0, sigma^2) for every link, from the stream above */
+void gfo_gaussian_momenta(const int *dims, double sigma, uint64_t seed, uint64_t sweep, double *P) {
+    long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
+    const double two_pi = 2.0 * GFO_PI;
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < V; s++)
+        for (int mu = 0; mu < 4; mu++) {
+            double *p = P + 8 * ((long)mu * V + s);
+            for (int b = 0; b < 4; b++) {
+                uint32_t w[4];
+                link_block(seed, sweep, s, mu, 1, b, w);
+                const double r = sqrt(-2.0 * log(unit53(w[0], w[1]))), ph = two_pi * unit53(w[2], w[3]);
+                p[2 * b] = sigma * r * cos(ph);
+                p[2 * b + 1] = sigma * r * sin(ph);
+            }
+        }
+}
+
+/* Hot start (randomGaugefields / hot condition, gaugefields_4D_wing.jl:295-322 with
+ * normalize3!, AbstractGaugefields.jl:3378-3457): entries (u - 1/2) + i (u' - 1/2), element
+ * (i, j) from uniforms 2(3i + j), 2(3i + j) + 1; rows 0 and 1 Gram-Schmidt normalized, row 2 =
+ * conj(row0 x row1). */
+void gfo_hot_links(const int *dims, uint64_t seed, double *U) {
+    long V = (long)dims[0] * dims[1] * dims[2] * dims[3];
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < V; s++)
+        for (int mu = 0; mu < 4; mu++) {
+            double u[20];
+            for (int b = 0; b < 5; b++) {
+                uint32_t w[4];
+                link_block(seed, 0, s, mu, 2, b, w);
+                for (int k = 0; k < 4; k++) u[4 * b + k] = unit32(w[k]);
+            }
+            cplx r0[3], r1[3];
+            for (int j = 0; j < 3; j++) {
+                r0[j] = CMPLX(u[2 * j] - 0.5, u[2 * j + 1] - 0.5);
+                r1[j] = CMPLX(u[2 * (3 + j)] - 0.5, u[2 * (3 + j) + 1] - 0.5);
+            }
+            double n0 = 0.0;
+            for (int j = 0; j < 3; j++) n0 += creal(r0[j] * conj(r0[j]));
+            n0 = 1.0 / sqrt(n0);
+            for (int j = 0; j < 3; j++) r0[j] *= n0;
+            cplx d = 0.0;
+            for (int j = 0; j < 3; j++) d += r1[j] * conj(r0[j]);
+            for (int j = 0; j < 3; j++) r1[j] -= d * r0[j];
+            double n1 = 0.0;
+            for (int j = 0; j < 3; j++) n1 += creal(r1[j] * conj(r1[j]));
+            n1 = 1.0 / sqrt(n1);
+            for (int j = 0; j < 3; j++) r1[j] *= n1;
+            m3 m;
+            for (int j = 0; j < 3; j++) {
+                m.e[0][j] = r0[j];
+                m.e[1][j] = r1[j];
+            }
+            for (int k = 0; k < 3; k++) {
+                const int a = (k + 1) % 3, b = (k + 2) % 3;
+                m.e[2][k] = conj(r0[a] * r1[b] - r0[b] * r1[a]);
+            }
+            st(U + 18 * ((long)mu * V + s), m);
+        }
 }

@@ -19,6 +19,8 @@
 #ifndef GF_ORACLE_H
 #define GF_ORACLE_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -54,6 +56,19 @@ void gfo_stout_smear(const int *dims, const double *U, int nlev, double rho, dou
 void gfo_stout_force(const int *dims, const double *U, int nlev, double rho, double coeff,
                      double *F);
 double gfo_stout_potential(const int *dims, const double *U, int nlev, double rho, double coeff);
+
+/* topological charge methods (topological_charge(U; method), AbstractGaugefields.jl:1430-1471)
+ * plus the rectangle density the improved one is built from */
+enum { GFO_TOPO_PLAQUETTE = 0, GFO_TOPO_CLOVER = 1, GFO_TOPO_IMPROVED = 2, GFO_TOPO_RECTANGLE = 3 };
+/* density q[V] (may be NULL); returns Q = sum q */
+double gfo_topological_charge_density(const int *dims, const double *U, int method, double *q);
+/* clover energy density (samples/measurements/energydensity.jl:32-76) */
+double gfo_energy_density_clover(const int *dims, const double *U);
+
+/* counter-based random fields: Philox4x32-10 and the per-link streams built on it */
+void gfo_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+void gfo_gaussian_momenta(const int *dims, double sigma, uint64_t seed, uint64_t sweep, double *P);
+void gfo_hot_links(const int *dims, uint64_t seed, double *U);
 
 /* integrator: 0 = QPQ, 1 = PQP, 2 = Omelyan 2MN (lambda argument) */
 void gfo_md_trajectory(const int *dims, double *U, double *P, double coeff, int nlev,

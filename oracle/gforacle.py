@@ -50,6 +50,12 @@ def lib():
             "gfo_stout_smear": (None, [_I, _D, ctypes.c_int, ctypes.c_double, _D]),
             "gfo_stout_force": (None, [_I, _D, ctypes.c_int, ctypes.c_double, ctypes.c_double, _D]),
             "gfo_stout_potential": (ctypes.c_double, [_I, _D, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+            "gfo_topological_charge_density": (ctypes.c_double, [_I, _D, ctypes.c_int, _D]),
+            "gfo_energy_density_clover": (ctypes.c_double, [_I, _D]),
+            "gfo_philox4x32_10": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                         ctypes.POINTER(ctypes.c_uint32)]),
+            "gfo_gaussian_momenta": (None, [_I, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64, _D]),
+            "gfo_hot_links": (None, [_I, ctypes.c_uint64, _D]),
             "gfo_md_trajectory": (None, [_I, _D, _D, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, _D]),
         }
@@ -175,6 +181,20 @@ def energy_density(dims, U):
     return lib().gfo_energy_density(_dims(dims), _p(_links(U)))
 
 
+TOPO_METHODS = {"plaquette": 0, "clover": 1, "improved": 2, "rectangle": 3}
+
+
+def topological_charge_density(dims, U, method="plaquette"):
+    """(Q, q) with q the per-site density in site order x + NX(y + NY(z + NZ t))"""
+    q = np.zeros(int(np.prod(dims)))
+    Q = lib().gfo_topological_charge_density(_dims(dims), _p(_links(U)), TOPO_METHODS[method], _p(q))
+    return Q, q
+
+
+def energy_density_clover(dims, U):
+    return lib().gfo_energy_density_clover(_dims(dims), _p(_links(U)))
+
+
 def stout_forward(dims, U, rho, want_C=False):
     U = _links(U)
     out = np.zeros_like(U)
@@ -218,3 +238,23 @@ def md_trajectory(dims, U, P, coeff, steps, tau, integrator="QPQ", lam=0.1931833
     lib().gfo_md_trajectory(_dims(dims), _p(U), _p(P), float(coeff), int(nlev), float(rho),
                             INTEGRATORS[integrator], float(lam), int(steps), float(tau), _p(H))
     return tuple(H)
+
+
+def philox4x32_10(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().gfo_philox4x32_10(c, k, o)
+    return tuple(o)
+
+
+def gaussian_momenta(dims, seed, sweep=0, sigma=1.0):
+    P = np.zeros((4, int(np.prod(dims)), 8))
+    lib().gfo_gaussian_momenta(_dims(dims), float(sigma), int(seed), int(sweep), _p(P))
+    return P
+
+
+def hot_links(dims, seed):
+    U = np.zeros((4, int(np.prod(dims)), 9), np.complex128)
+    lib().gfo_hot_links(_dims(dims), int(seed), _p(U))
+    return U

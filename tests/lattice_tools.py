@@ -130,3 +130,96 @@ def perturb_link(U, mu, s, a, h):
     E = expm(h * 1j * T_GEN[a])
     M[mu, s] = E @ M[mu, s]
     return gen.matrices_to_links(M)
+
+
+# ---------------------------------------------------------------- field strengths
+
+
+def path_field(M, path):
+    """Whole-lattice product of links along path [(mu, n), ...] from every site x:
+    (mu, +n) takes U_mu(y) and moves y -> y + mu, n times; (mu, -n) moves y -> y - mu and
+    takes U_mu(y)^dag, n times.  M: (4, NT, NZ, NY, NX, 3, 3)."""
+    disp = [0, 0, 0, 0]
+    out = None
+    for mu, n in path:
+        for _ in range(abs(n)):
+            if n > 0:
+                A = M[mu]
+                for d in range(4):
+                    A = sh(A, d, disp[d]) if disp[d] else A
+                disp[mu] += 1
+            else:
+                disp[mu] -= 1
+                A = M[mu]
+                for d in range(4):
+                    A = sh(A, d, disp[d]) if disp[d] else A
+                A = dg(A)
+            out = A if out is None else out @ A
+    return out
+
+
+def _loops(method, m, n):
+    if method == "plaquette":
+        return [[(m, 1), (n, 1), (m, -1), (n, -1)]]
+    if method == "clover":
+        return [[(m, 1), (n, 1), (m, -1), (n, -1)], [(n, 1), (m, -1), (n, -1), (m, 1)],
+                [(n, -1), (m, 1), (n, 1), (m, -1)], [(m, -1), (n, -1), (m, 1), (n, 1)]]
+    return [[(m, 2), (n, 1), (m, -2), (n, -1)], [(n, 1), (m, -2), (n, -1), (m, 2)],
+            [(n, -1), (m, 2), (n, 1), (m, -2)], [(m, -2), (n, -1), (m, 2), (n, 1)],
+            [(m, 1), (n, 2), (m, -1), (n, -2)], [(n, 2), (m, -1), (n, -2), (m, 1)],
+            [(n, -2), (m, 1), (n, 2), (m, -1)], [(m, -1), (n, -2), (m, 1), (n, 2)]]
+
+
+def field_strengths(U, dims, method):
+    M = field(U, dims)
+    F = {}
+    for m in range(4):
+        for n in range(4):
+            if m != n:
+                F[m, n] = ta_matrix(sum(path_field(M, p) for p in _loops(method, m, n)))
+    return F
+
+
+def _levi_civita(p):
+    if len(set(p)) < 4:
+        return 0
+    inv = sum(1 for i in range(4) for j in range(i + 1, 4) if p[i] > p[j])
+    return -1 if inv % 2 else 1
+
+
+def topo_density_numpy(U, dims, method):
+    """density in site order, literal epsilon sum over all index quadruples"""
+    if method == "improved":
+        return (5 / 3) * topo_density_numpy(U, dims, "clover") - (1 / 12) * topo_density_numpy(U, dims, "rectangle")
+    nl = {"plaquette": 1, "clover": 4, "rectangle": 8}[method]
+    norm = 2.0 if method == "rectangle" else 1.0
+    F = field_strengths(U, dims, method)
+    q = 0.0
+    import itertools
+    for p in itertools.permutations(range(4)):
+        e = _levi_civita(p)
+        q = q + e * np.einsum("...ij,...ji->...", F[p[0], p[1]], F[p[2], p[3]])
+    return (-norm * q.real / (32 * np.pi ** 2 * nl ** 2)).reshape(-1)
+
+
+def abelian_flux_links(dims, n01, n23):
+    """Constant abelian field strength: every (0,1) plaquette equals diag(e^{i t}, e^{-i t}, 1)
+    with t = 2 pi n01 / (NX NY), every (2,3) plaquette the same with t' = 2 pi n23 / (NZ NT),
+    all other plaquettes 1 (twisted boundary links carry the flux quantum)."""
+    NX, NY, NZ, NT = dims
+    ph = np.zeros((4, NT, NZ, NY, NX))
+    x = np.arange(NX)[None, None, None, :]
+    y = np.arange(NY)[None, None, :, None]
+    z = np.arange(NZ)[None, :, None, None]
+    t = np.arange(NT)[:, None, None, None]
+    th = 2 * np.pi * n01 / (NX * NY)
+    tp = 2 * np.pi * n23 / (NZ * NT)
+    ph[1] = th * x + 0 * (y + z + t)
+    ph[0] = np.where(x == NX - 1, -2 * np.pi * n01 * y / NY, 0.0) + 0 * (z + t)
+    ph[3] = tp * z + 0 * (x + y + t)
+    ph[2] = np.where(z == NZ - 1, -2 * np.pi * n23 * t / NT, 0.0) + 0 * (x + y)
+    M = np.zeros((4, NT, NZ, NY, NX, 3, 3), complex)
+    M[..., 0, 0] = np.exp(1j * ph)
+    M[..., 1, 1] = np.exp(-1j * ph)
+    M[..., 2, 2] = 1.0
+    return unfield(M), th, tp

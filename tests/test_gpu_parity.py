@@ -49,13 +49,30 @@ def maxdiff(a, b):
 
 
 def test_roundtrip_exact(Lat):
+    # the device keeps rows 0 and 1 of each SU(3) link and rebuilds row 2 as conj(row0 x row1)
     L = Lat()
     U = gen.hot_start_links(D4, 1)
     P = gen.gaussian_momenta(D4, 2)
     L.upload_links(U)
     L.upload_momenta(P)
-    assert np.array_equal(L.download_links(), U)
+    M, Md = gen.links_to_matrices(U), gen.links_to_matrices(L.download_links())
+    assert np.array_equal(Md[..., :2, :], M[..., :2, :])
+    assert np.abs(Md[..., 2, :] - M[..., 2, :]).max() < 1e-15
     assert np.array_equal(L.download_momenta(), P)
+
+
+def test_upload_rejects_non_su3(Lat, gf):
+    L = Lat()
+    U = gen.hot_start_links(D4, 3)
+    M = gen.links_to_matrices(U).copy()
+    M[1, 17, 2, 0] += 1e-3
+    with pytest.raises(gf.GFError, match="links must be SU\\(3\\)"):
+        L.upload_links(gen.matrices_to_links(M))
+    M[1, 17, 2, 0] = np.nan
+    with pytest.raises(gf.GFError, match="links must be SU"):
+        L.upload_links(gen.matrices_to_links(M))
+    # within the limit (links written in single precision) is accepted
+    L.upload_links(U.astype(np.complex64).astype(np.complex128))
 
 
 @pytest.mark.parametrize("dims", [D4, (2, 2, 2, 2), (4, 2, 3, 2), (2, 4, 2, 6), (8, 4, 6, 2)])

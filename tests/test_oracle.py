@@ -362,3 +362,117 @@ def test_md_stout_reversibility(oracle):
     oracle.md_trajectory(D4, U, P, COEFF, 3, -0.1, "2MN", nlev=2, rho=0.1)
     assert np.abs(U - U0).max() < 2e-12
     assert abs(H[2]) < 0.5
+
+
+# ---------------------------------------------------------------- field strengths
+TOPO = ["plaquette", "clover", "rectangle", "improved"]
+
+
+def test_topological_charge_cold_is_zero(oracle):
+    # test/sun_embedded_instanton.jl:316-336: all densities vanish on a cold start
+    U = lt.gen.cold_links(D4)
+    for m in TOPO:
+        Q, q = oracle.topological_charge_density(D4, U, m)
+        assert Q == 0.0 and np.all(q == 0.0)
+    assert oracle.energy_density_clover(D4, U) == 0.0
+
+
+@pytest.mark.parametrize("dims,n", [((4, 4, 4, 4), (1, 1)), ((4, 6, 6, 4), (2, -1)), ((8, 4, 4, 8), (1, 3))])
+def test_field_strength_constant_abelian_closed_form(oracle, dims, n):
+    # constant flux t through every (0,1) plaquette and t' through every (2,3) plaquette, as
+    # diag(e^{it}, e^{-it}, 1): the clover and plaquette F_01 = 4 or 1 times diag(i sin t,
+    # -i sin t, 0), so q = sin t sin t' / (2 pi^2) per site; the 2x1 rectangles carry 2t;
+    # E_clover = 2 (sin^2 t + sin^2 t'), E_plaq = 4 (1 - cos t) + 4 (1 - cos t')
+    U, t, tp = lt.abelian_flux_links(dims, *n)
+    V = int(np.prod(dims))
+    qc = np.sin(t) * np.sin(tp) / (2 * np.pi ** 2)
+    qr = np.sin(2 * t) * np.sin(2 * tp) / np.pi ** 2
+    expect = {"plaquette": qc, "clover": qc, "rectangle": qr, "improved": 5 / 3 * qc - 1 / 12 * qr}
+    for m in TOPO:
+        Q, q = oracle.topological_charge_density(dims, U, m)
+        assert np.allclose(q, expect[m], rtol=1e-12, atol=1e-15)
+        assert Q == pytest.approx(V * expect[m], rel=1e-12)
+    assert oracle.energy_density_clover(dims, U) == pytest.approx(2 * (np.sin(t) ** 2 + np.sin(tp) ** 2), rel=1e-12)
+    assert oracle.energy_density(dims, U) == pytest.approx(4 * (2 - np.cos(t) - np.cos(tp)), rel=1e-12)
+
+
+def test_abelian_charge_tends_to_flux_product(oracle):
+    # small flux quanta on a larger torus: the improved charge approaches its continuum value
+    # 2 n01 n23 (two U(1) charges of the diag(1, -1, 0) embedding)
+    dims = (12, 12, 12, 12)
+    U, _, _ = lt.abelian_flux_links(dims, 1, 1)
+    for m in ("clover", "improved"):
+        Q, _ = oracle.topological_charge_density(dims, U, m)
+        assert Q == pytest.approx(2.0, abs=2e-3)
+
+
+@pytest.mark.parametrize("dims", [(2, 2, 2, 2), (3, 2, 2, 3), (4, 3, 2, 3)])
+def test_topological_density_vs_numpy(oracle, dims):
+    U = hot(dims, 21)
+    for m in TOPO:
+        Q, q = oracle.topological_charge_density(dims, U, m)
+        qn = lt.topo_density_numpy(U, dims, m)
+        assert np.abs(q - qn).max() < 1e-13
+        assert Q == pytest.approx(qn.sum(), rel=1e-11, abs=1e-13)
+
+
+def test_topological_density_gauge_invariant(oracle):
+    rng = np.random.default_rng(8)
+    U = alg_links(oracle, D4, 0.4, 9)
+    Ug = lt.gauge_transform(U, D4, lt.random_su3(256, rng))
+    for m in TOPO:
+        q1 = oracle.topological_charge_density(D4, U, m)[1]
+        q2 = oracle.topological_charge_density(D4, Ug, m)[1]
+        assert np.abs(q1 - q2).max() < 1e-13
+    assert oracle.energy_density_clover(D4, Ug) == pytest.approx(oracle.energy_density_clover(D4, U), rel=1e-13)
+
+
+def test_clover_charge_odd_under_reflection(oracle):
+    # x0 -> -x0: U'_0(x) = U_0(Rx - 0)^dag, U'_k(x) = U_k(Rx); the clover and improved
+    # charges change sign, the clover energy does not
+    dims = (4, 4, 2, 3)
+    U = alg_links(oracle, dims, 0.5, 12)
+    M = lt.field(U, dims)  # (4, NT, NZ, NY, NX, 3, 3)
+    R = np.flip(M, axis=4)
+    R = np.roll(R, 1, axis=4)  # index x holds M[(-x) mod NX]
+    out = R.copy()
+    out[0] = lt.dg(np.roll(R[0], -1, axis=3))  # U_0((-x-1) mod NX)^dag
+    Ur = lt.unfield(out)
+    for m in ("clover", "improved"):
+        assert oracle.topological_charge_density(dims, Ur, m)[0] == pytest.approx(
+            -oracle.topological_charge_density(dims, U, m)[0], rel=1e-12, abs=1e-14)
+    assert oracle.energy_density_clover(dims, Ur) == pytest.approx(oracle.energy_density_clover(dims, U), rel=1e-12)
+
+
+def test_instanton_charge_independent_of_embedding(oracle):
+    # test/sun_embedded_instanton.jl:376-391: the SU(2) instanton embedded at blocks (1,2),
+    # (1,3), (2,3) has one charge; |Q| > 0.1 (:341)
+    vals = []
+    for blk in [(0, 1), (0, 2), (1, 2)]:
+        U = lt.gen.matrices_to_links(lt.instanton_links(D4, 3, blk))
+        vals.append([oracle.topological_charge_density(D4, U, m)[0] for m in ("plaquette", "clover", "improved")])
+    assert np.allclose(vals, vals[0], rtol=1e-12, atol=1e-14)
+    assert abs(vals[0][0]) > 0.1
+
+
+# ---------------------------------------------------------------- random fields
+def test_philox_known_answers(oracle):
+    # Philox4x32-10 known-answer vectors of the Random123 distribution (kat_vectors)
+    assert oracle.philox4x32_10((0, 0, 0, 0), (0, 0)) == (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)
+    assert oracle.philox4x32_10((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2) == (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)
+    assert oracle.philox4x32_10((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0)) == (
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)
+
+
+def test_oracle_random_fields(oracle):
+    dims = (6, 6, 6, 6)
+    P = oracle.gaussian_momenta(dims, 77, 0)
+    n = P.size
+    assert abs(P.mean()) < 5 / np.sqrt(n) and abs(P.var() - 1) < 5 * np.sqrt(2 / n)
+    assert np.array_equal(P, oracle.gaussian_momenta(dims, 77, 0))
+    assert not np.array_equal(P, oracle.gaussian_momenta(dims, 77, 1))
+    assert np.allclose(oracle.gaussian_momenta(dims, 77, 0, sigma=2.0), 2.0 * P, rtol=0, atol=1e-15)
+    U = oracle.hot_links(dims, 5)
+    M = lt.gen.links_to_matrices(U)
+    assert np.abs(M @ np.conj(np.swapaxes(M, -1, -2)) - np.eye(3)).max() < 1e-14
+    assert np.abs(np.linalg.det(M) - 1).max() < 1e-14
