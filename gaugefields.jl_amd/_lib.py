@@ -40,6 +40,9 @@ SIGNATURES = {
     "gf_restore_links": (ctypes.c_int, [_CTX]),
     "gf_gaussian_momenta": (ctypes.c_int, [_CTX, ctypes.c_double, ctypes.c_ulonglong, ctypes.c_ulonglong]),
     "gf_hot_start": (ctypes.c_int, [_CTX, ctypes.c_ulonglong]),
+    "gf_heatbath": (ctypes.c_int, [_CTX, ctypes.c_double, ctypes.c_ulonglong, ctypes.c_ulonglong, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_longlong)]),
+    "gf_overrelaxation": (ctypes.c_int, [_CTX, ctypes.c_ulonglong, ctypes.c_ulonglong]),
     "gf_plaquette": (ctypes.c_int, [_CTX, _D]),
     "gf_action": (ctypes.c_int, [_CTX, ctypes.c_double, _D]),
     "gf_momenta_dot": (ctypes.c_int, [_CTX, _D]),

@@ -18,6 +18,7 @@ are the same names without ``!``; they mutate their first argument in place):
   stout_smearing / smear / back_prop   API.jl:452-480, Abstractsmearing.jl:237-411
   topological_charge(_density)  AbstractGaugefields.jl:1430-1471
   calculate_energy_density   samples/measurements/energydensity.jl:32-76 (clover)
+  Heatbath / Heatbath_update / heatbath / overrelaxation   heatbath/heatbathmodule.jl:50-200, 539-602, 2119-2163
   ILDG / load_gaugefield / save_binarydata     ildg_format.jl:686-695, 797-833, 902-969
   save_configuration / load_configuration_     API.jl:711-727, 781-826 (format=:ildg)
 
@@ -73,6 +74,10 @@ __all__ = [
     "topological_charge",
     "topological_charge_density",
     "calculate_energy_density",
+    "Heatbath",
+    "Heatbath_update",
+    "heatbath",
+    "overrelaxation",
     "ILDG",
     "load_gaugefield",
     "save_binarydata",
@@ -643,3 +648,48 @@ def topological_charge_density(U, *, method="plaquette"):
 def calculate_energy_density(U):
     """clover energy density (calculate_energy_density, samples/measurements/energydensity.jl:68-76)"""
     return U.lattice.energy_density_clover()
+
+
+# ----------------------------------------------------------------------------- heatbath
+
+
+class Heatbath:
+    """Heatbath(U, beta; ITERATION_MAX, seed, sweep, overrelaxation_sweep) (heatbathmodule.jl:50-97):
+    checkerboard SU(3) heatbath / overrelaxation of the Wilson plaquette action at beta.
+    Successful heatbath / overrelaxation calls advance sweep / overrelaxation_sweep."""
+
+    def __init__(self, U, beta, *, ITERATION_MAX=10 ** 5, seed=0, sweep=0, overrelaxation_sweep=None):
+        if ITERATION_MAX <= 0:
+            raise ArgumentError("ITERATION_MAX must be positive")
+        if ITERATION_MAX >= 0xFFFF:
+            raise ArgumentError("ITERATION_MAX must be below 65535 on the HIP backend (one counter block per try)")
+        if any(int(d) % 2 for d in U.dims):
+            raise ArgumentError(f"the checkerboard heatbath needs even lattice extents; got {list(U.dims)}")
+        self.beta = float(beta)
+        self.ITERATION_MAX = int(ITERATION_MAX)
+        self.seed = int(seed)
+        self.sweep = int(sweep)
+        self.overrelaxation_sweep = int(sweep if overrelaxation_sweep is None else overrelaxation_sweep)
+        self.failures = 0
+
+
+def Heatbath_update(U, gauge_action, *, ITERATION_MAX=10 ** 5, seed=0, sweep=0, overrelaxation_sweep=None):
+    """Heatbath_update(U, gauge_action; ...) (heatbathmodule.jl:148-200) for the plaquette action
+    push!(action, beta/2, plaquette ∪ plaquette'): the same updater at beta = 2 * coefficient"""
+    return Heatbath(U, 2.0 * gauge_action.coefficient, ITERATION_MAX=ITERATION_MAX, seed=seed, sweep=sweep,
+                    overrelaxation_sweep=overrelaxation_sweep)
+
+
+def heatbath(U, h):
+    """heatbath!(U, h) (heatbathmodule.jl:2165-2181): one sweep, then h.sweep += 1.  Links whose
+    SU(2) proposals were all rejected within ITERATION_MAX tries stay unchanged (h.failures)."""
+    h.failures += U.lattice.heatbath(h.beta, h.seed, h.sweep, h.ITERATION_MAX)
+    h.sweep += 1
+    return U
+
+
+def overrelaxation(U, h):
+    """overrelaxation!(U, h): one microcanonical sweep, then h.overrelaxation_sweep += 1"""
+    U.lattice.overrelaxation(h.seed, h.overrelaxation_sweep)
+    h.overrelaxation_sweep += 1
+    return U

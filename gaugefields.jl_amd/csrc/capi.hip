@@ -340,7 +340,7 @@ int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
         if (hipEventCreate(&c->ev[i]) != hipSuccess) return bail(GF_EDEVICE);
     if (alloc(c, &c->U, link_bytes(c)) || alloc(c, &c->Ualt, link_bytes(c)) || alloc(c, &c->P, mom_bytes(c)) ||
         alloc(c, &c->F, mom_bytes(c)) ||
-        alloc(c, (void**)&c->partial, sizeof(double) * 4 * std::max(grid_for(c->V), grid_sites_mu(c->V))) ||
+        alloc(c, (void**)&c->partial, sizeof(double) * (4 * std::max(grid_for(c->V), grid_sites_mu(c->V)) + 16)) ||
         alloc(c, (void**)&c->dres, 4 * sizeof(double)))
         return bail(GF_ENOMEM);
     if (hipHostMalloc((void**)&c->hres, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) return bail(GF_ENOMEM);
@@ -677,6 +677,37 @@ int gf_hot_start(gf_ctx* c, unsigned long long seed) {
         HIPCHK(c, launch_hot_links<T>(c->geo, (T*)c->U, seed, c->stream));
         return GF_OK;
     });
+}
+
+static int heatbath_sweep(gf_ctx* c, int mode, double beta, unsigned long long seed, unsigned long long sweep,
+                          int itmax, long long* failures) {
+    for (int m = 0; m < 4; m++)
+        if (c->geo.n[m] % 2)
+            return fail(c, GF_EINVAL, "the checkerboard heatbath needs even lattice extents; got %d in direction %d",
+                        c->geo.n[m], m + 1);
+    if (itmax <= 0) return fail(c, GF_EINVAL, "ITERATION_MAX must be positive");
+    if (itmax >= 0xFFFF) return fail(c, GF_EINVAL, "ITERATION_MAX must be below 65535 (one counter block per try)");
+    if (mode == 0 && !(beta > 0.0 && std::isfinite(beta))) return fail(c, GF_EINVAL, "beta must be positive; got %g", beta);
+    TRY(dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_heatbath<T>(c->geo, (T*)c->U, mode, beta, seed, sweep, itmax, c->partial, c->dres, c->stream));
+        return GF_OK;
+    }));
+    double f;
+    TRY(read_scalar(c, &f));
+    if (failures) *failures = (long long)f;
+    return GF_OK;
+}
+
+int gf_heatbath(gf_ctx* c, double beta, unsigned long long seed, unsigned long long sweep, int itmax,
+                long long* failures) {
+    CTXCHK(c);
+    return heatbath_sweep(c, 0, beta, seed, sweep, itmax, failures);
+}
+
+int gf_overrelaxation(gf_ctx* c, unsigned long long seed, unsigned long long sweep) {
+    CTXCHK(c);
+    return heatbath_sweep(c, 1, 0.0, seed, sweep, 1, nullptr);
 }
 
 int gf_topological_charge(gf_ctx* c, int method, double* Q) {

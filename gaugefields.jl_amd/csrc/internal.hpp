@@ -74,6 +74,12 @@ hipError_t launch_gaussian_momenta(const Geo& g, T* P, double sigma, unsigned lo
 template <typename T>
 hipError_t launch_hot_links(const Geo& g, T* U, unsigned long long seed, hipStream_t st);
 
+// heatbath (mode 0) or overrelaxation (mode 1) sweep: mu = 0..3, colours even then odd;
+// failures (device double) = links a heatbath left unchanged.  partial: >= 8 * (V/2 + 255)/256
+template <typename T>
+hipError_t launch_heatbath(const Geo& g, T* U, int mode, double beta, unsigned long long seed,
+                           unsigned long long sweep, int itmax, double* partial, double* failures, hipStream_t st);
+
 // ---- k_flow.hip ----
 // A = alpha * TA(W V^dag) + beta * A ; Wout = exp(A) W
 template <typename T>

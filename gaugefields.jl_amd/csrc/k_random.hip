@@ -115,3 +115,249 @@ GF_INST_RNG(float)
 #undef GF_INST_RNG
 
 }  // namespace gf
+
+// ===========================================================================
+// Heatbath and overrelaxation (Cabibbo-Marinari over the SU(2) subgroups (1,2), (2,3), (1,3)).
+// One thread per site of one checkerboard colour for one direction mu: the staple of U_mu(x)
+// only touches U_mu at the other colour and the other directions, so the colour updates in
+// place.  Streams (same counter layout as above): tags 3, 4, 5 = heatbath subgroups (block =
+// try index, 0xFFFF = the two draws after acceptance), tag 6 = overrelaxation subgroup choice.
+// Everything runs in double; FP32 links are rounded once when stored.
+// ===========================================================================
+namespace gf {
+
+namespace {
+
+struct C2 {
+    double r, i;
+};
+__device__ __forceinline__ C2 c2(double r, double i) { return C2{r, i}; }
+__device__ __forceinline__ C2 cmul2(C2 a, C2 b) { return C2{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ C2 cconj2(C2 a) { return C2{a.r, -a.i}; }
+__device__ __forceinline__ C2 cadd2(C2 a, C2 b) { return C2{a.r + b.r, a.i + b.i}; }
+__device__ __forceinline__ C2 cscale2(double s, C2 a) { return C2{s * a.r, s * a.i}; }
+__device__ __forceinline__ C2 cdiv2(C2 a, C2 b) {
+    const double d = b.r * b.r + b.i * b.i;
+    return C2{(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
+}
+struct M2 {
+    C2 e[2][2];
+};
+
+// project_onto_SU2! (heatbath/portable/kernels.jl:12-23)
+__device__ __forceinline__ M2 su2_project(const M2& S) {
+    const C2 a = cadd2(cscale2(0.5, S.e[0][0]), cscale2(0.5, cconj2(S.e[1][1])));
+    const C2 b = cadd2(cscale2(0.5, S.e[1][0]), cscale2(-0.5, cconj2(S.e[0][1])));
+    M2 r;
+    r.e[0][0] = a;
+    r.e[1][0] = b;
+    r.e[0][1] = cscale2(-1.0, cconj2(b));
+    r.e[1][1] = cconj2(a);
+    return r;
+}
+
+template <int n, int k>
+__device__ __forceinline__ M2 sub2(const M3<double>& m) {
+    M2 S;
+    S.e[0][0] = c2(m.m[3 * n + n].r, m.m[3 * n + n].i);
+    S.e[0][1] = c2(m.m[3 * n + k].r, m.m[3 * n + k].i);
+    S.e[1][0] = c2(m.m[3 * k + n].r, m.m[3 * k + n].i);
+    S.e[1][1] = c2(m.m[3 * k + k].r, m.m[3 * k + k].i);
+    return S;
+}
+
+// u <- embed(K at rows/columns n, k) u
+template <int n, int k>
+__device__ __forceinline__ void su2_left_mul(const M2& K, M3<double>& u) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const C2 a = c2(u.m[3 * n + j].r, u.m[3 * n + j].i), b = c2(u.m[3 * k + j].r, u.m[3 * k + j].i);
+        const C2 na = cadd2(cmul2(K.e[0][0], a), cmul2(K.e[0][1], b));
+        const C2 nb = cadd2(cmul2(K.e[1][0], a), cmul2(K.e[1][1], b));
+        u.m[3 * n + j].r = na.r;
+        u.m[3 * n + j].i = na.i;
+        u.m[3 * k + j].r = nb.r;
+        u.m[3 * k + j].i = nb.i;
+    }
+}
+
+// Kennedy-Pendleton SU(2) heatbath (_su2_update_kp_core!, heatbath/portable/kernels.jl:63-150)
+__device__ __forceinline__ bool kp_update(unsigned long long seed, unsigned long long sweep, unsigned s, int mu,
+                                          int tag, double beta, int itmax, const M2& S, M2& K) {
+    const double rho0 = (S.e[0][0].r + S.e[1][1].r) * 0.5, rho1 = -(S.e[0][1].i + S.e[1][0].i) * 0.5;
+    const double rho2 = (S.e[1][0].r - S.e[0][1].r) * 0.5, rho3 = (S.e[1][1].i - S.e[0][0].i) * 0.5;
+    const double rho = sqrt(rho0 * rho0 + rho1 * rho1 + rho2 * rho2 + rho3 * rho3);
+    const C2 det = cadd2(cmul2(S.e[0][0], S.e[1][1]), cscale2(-1.0, cmul2(S.e[0][1], S.e[1][0])));
+    M2 V0;
+    V0.e[0][0] = cdiv2(cscale2(rho, S.e[1][1]), det);
+    V0.e[0][1] = cdiv2(cscale2(-rho, S.e[0][1]), det);
+    V0.e[1][0] = cdiv2(cscale2(-rho, S.e[1][0]), det);
+    V0.e[1][1] = cdiv2(cscale2(rho, S.e[0][0]), det);
+    const double k = 2.0 * (beta / 3.0) * rho;
+    double delta = 0.0;
+    bool ok = false;
+    for (int t = 0; t < itmax && !ok; t++) {
+        unsigned w[4];
+        link_block(seed, sweep, s, mu, tag, t, w);
+        const double X = -log(unit32(w[0])) / k, Xp = -log(unit32(w[1])) / k;
+        const double c = cos(2.0 * 3.14159265358979323846 * unit32(w[2]));
+        delta = Xp + X * c * c;
+        const double r3 = unit32(w[3]);
+        ok = r3 * r3 <= 1.0 - 0.5 * delta;
+    }
+    if (!ok) return false;
+    unsigned w[4];
+    link_block(seed, sweep, s, mu, tag, 0xFFFF, w);
+    const double a1 = 1.0 - delta, rr = sqrt(1.0 - a1 * a1);
+    const double phi = 2.0 * 3.14159265358979323846 * unit32(w[0]);
+    const double ct = 2.0 * (unit32(w[1]) - 0.5), st = sqrt(1.0 - ct * ct);
+    const double a2 = rr * cos(phi) * st, a3 = rr * sin(phi) * st, a4 = rr * ct;
+    M2 tmp;
+    tmp.e[0][0] = c2(a1, a4);
+    tmp.e[0][1] = c2(a3, a2);
+    tmp.e[1][0] = c2(-a3, a2);
+    tmp.e[1][1] = c2(a1, -a4);
+    M2 Un;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) Un.e[i][j] = cadd2(cmul2(tmp.e[i][0], V0.e[0][j]), cmul2(tmp.e[i][1], V0.e[1][j]));
+    const C2 al = cadd2(cscale2(0.5, Un.e[0][0]), cscale2(0.5, cconj2(Un.e[1][1])));
+    const C2 be = cadd2(cscale2(0.5, Un.e[1][0]), cscale2(-0.5, cconj2(Un.e[0][1])));
+    const double d = al.r * al.r + al.i * al.i + be.r * be.r + be.i * be.i;
+    K.e[0][0] = cscale2(1.0 / d, al);
+    K.e[1][0] = cscale2(1.0 / d, be);
+    K.e[0][1] = cscale2(-1.0 / d, cconj2(be));
+    K.e[1][1] = cscale2(1.0 / d, cconj2(al));
+    return true;
+}
+
+// overrelaxation subgroup step (_overrelaxation_subgroup!, heatbathmodule.jl:1248-1280)
+template <int n, int k>
+__device__ __forceinline__ void or_subgroup(M3<double>& u, const M3<double>& V) {
+    const M2 w = su2_project(sub2<n, k>(mul(u, V)));
+    M2 h;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            h.e[i][j] = cadd2(cmul2(cconj2(w.e[0][i]), cconj2(w.e[j][0])), cmul2(cconj2(w.e[1][i]), cconj2(w.e[j][1])));
+    // Gram-Schmidt on the columns
+    {
+        const double n0 = 1.0 / sqrt(h.e[0][0].r * h.e[0][0].r + h.e[0][0].i * h.e[0][0].i + h.e[1][0].r * h.e[1][0].r +
+                                     h.e[1][0].i * h.e[1][0].i);
+        h.e[0][0] = cscale2(n0, h.e[0][0]);
+        h.e[1][0] = cscale2(n0, h.e[1][0]);
+        const C2 p = cadd2(cmul2(cconj2(h.e[0][0]), h.e[0][1]), cmul2(cconj2(h.e[1][0]), h.e[1][1]));
+        h.e[0][1] = cadd2(h.e[0][1], cscale2(-1.0, cmul2(p, h.e[0][0])));
+        h.e[1][1] = cadd2(h.e[1][1], cscale2(-1.0, cmul2(p, h.e[1][0])));
+        const double n1 = 1.0 / sqrt(h.e[0][1].r * h.e[0][1].r + h.e[0][1].i * h.e[0][1].i + h.e[1][1].r * h.e[1][1].r +
+                                     h.e[1][1].i * h.e[1][1].i);
+        h.e[0][1] = cscale2(n1, h.e[0][1]);
+        h.e[1][1] = cscale2(n1, h.e[1][1]);
+    }
+    su2_left_mul<n, k>(h, u);
+}
+
+// one heatbath subgroup step; false (u unchanged) if no proposal was accepted
+template <int n, int k>
+__device__ __forceinline__ bool hb_subgroup(M3<double>& u, const M3<double>& St, unsigned long long seed,
+                                            unsigned long long sweep, unsigned s, int mu, int tag, double beta,
+                                            int itmax) {
+    M2 K;
+    if (!kp_update(seed, sweep, s, mu, tag, beta, itmax, su2_project(sub2<n, k>(mul(u, St))), K)) return false;
+    su2_left_mul<n, k>(K, u);
+    return true;
+}
+
+// site of colour `par` with half-volume index h (NX even): x = 2 (h mod NX/2) + ((y+z+t+par) & 1)
+__device__ __forceinline__ int colour_site(const Geo& g, int h, int par, int x[4]) {
+    const int nxh = g.n[0] >> 1;
+    int r = h / nxh;
+    x[1] = r % g.n[1];
+    r /= g.n[1];
+    x[2] = r % g.n[2];
+    x[3] = r / g.n[2];
+    x[0] = 2 * (h % nxh) + ((x[1] + x[2] + x[3] + par) & 1);
+    return x[0] + g.n[0] * (x[1] + g.n[1] * (x[2] + g.n[2] * x[3]));
+}
+
+}  // namespace
+
+// MODE 0: heatbath (SU3update_matrix!, heatbath/portable/kernels.jl:199-300; a rejected
+// subgroup leaves the link unchanged and counts a failure); MODE 1: overrelaxation
+// (SUN_overrelaxation_rng!, heatbathmodule.jl:1295-1321, subgroup i drawn from block i of tag 6:
+// n = floor(2 w0 / 2^32), k = n + 1 + floor((2 - n) w1 / 2^32)).
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_heatbath(Geo g, T* __restrict__ U, int mu, int par, double beta,
+                                                     unsigned long long seed, unsigned long long sweep, int itmax,
+                                                     double* __restrict__ partial) {
+    const int h = (int)(xcd_block() * blockDim.x + threadIdx.x);
+    double fail = 0.0;
+    if (h < (g.V >> 1)) {
+        int x[4];
+        const int s = colour_site(g, h, par, x);
+        const unsigned V = g.V;
+        const M3<double> St = to_f64(staple_sigma(g, (const T*)U, V, s, x, mu));
+        M3<double> u = to_f64(ldu((const T*)U, V, mu, (unsigned)s));
+        bool ok = true;
+        if (MODE == 0) {
+            ok = hb_subgroup<0, 1>(u, St, seed, sweep, (unsigned)s, mu, 3, beta, itmax) &&
+                 hb_subgroup<1, 2>(u, St, seed, sweep, (unsigned)s, mu, 4, beta, itmax) &&
+                 hb_subgroup<0, 2>(u, St, seed, sweep, (unsigned)s, mu, 5, beta, itmax);
+        } else {
+#pragma unroll 1
+            for (int i = 0; i < 3; i++) {
+                unsigned w[4];
+                link_block(seed, sweep, (unsigned)s, mu, 6, i, w);
+                const int n = (int)(((unsigned long long)w[0] * 2ull) >> 32);
+                const int k = n + 1 + (int)(((unsigned long long)w[1] * (unsigned long long)(2 - n)) >> 32);
+                if (n == 1)
+                    or_subgroup<1, 2>(u, St);
+                else if (k == 1)
+                    or_subgroup<0, 1>(u, St);
+                else
+                    or_subgroup<0, 2>(u, St);
+            }
+        }
+        if (ok) {
+            reunitarize_f64(u);
+            M3<T> ut;
+#pragma unroll
+            for (int e = 0; e < 9; e++) {
+                ut.m[e].r = (T)u.m[e].r;
+                ut.m[e].i = (T)u.m[e].i;
+            }
+            stu<T>(U, V, mu, (unsigned)s, ut);
+        } else {
+            fail = 1.0;
+        }
+    }
+    block_sum_store<kBlock>(fail, partial);
+}
+
+template <typename T>
+hipError_t launch_heatbath(const Geo& g, T* U, int mode, double beta, unsigned long long seed,
+                           unsigned long long sweep, int itmax, double* partial, double* failures, hipStream_t st) {
+    const int nb = grid_for(g.V / 2);
+    for (int mu = 0; mu < 4; mu++)
+        for (int par = 0; par < 2; par++) {
+            double* part = partial + (size_t)(2 * mu + par) * nb;
+            if (mode == 0)
+                k_heatbath<T, 0><<<nb, kBlock, 0, st>>>(g, U, mu, par, beta, seed, sweep, itmax, part);
+            else
+                k_heatbath<T, 1><<<nb, kBlock, 0, st>>>(g, U, mu, par, beta, seed, sweep, itmax, part);
+        }
+    hipError_t err = hipGetLastError();
+    if (err == hipSuccess) err = launch_sum_partials(partial, 8 * nb, failures, st);
+    return err;
+}
+
+#define GF_INST_HB(T)                                                                                      \
+    template hipError_t launch_heatbath<T>(const Geo&, T*, int, double, unsigned long long, unsigned long long, \
+                                           int, double*, double*, hipStream_t);
+GF_INST_HB(double)
+GF_INST_HB(float)
+#undef GF_INST_HB
+
+}  // namespace gf

@@ -140,6 +140,17 @@ class Lattice:
         """random SU(3) links generated on the device (Philox4x32-10 stream seed)"""
         self._c(self._L.gf_hot_start(self._ctx, ctypes.c_ulonglong(seed)))
 
+    def heatbath(self, beta, seed, sweep, itmax=100000):
+        """one checkerboard heatbath sweep (plaquette action at beta); returns the number of
+        links left unchanged because no proposal was accepted within itmax tries"""
+        f = ctypes.c_longlong()
+        self._c(self._L.gf_heatbath(self._ctx, ctypes.c_double(beta), ctypes.c_ulonglong(seed),
+                                    ctypes.c_ulonglong(sweep), int(itmax), ctypes.byref(f)))
+        return f.value
+
+    def overrelaxation(self, seed, sweep):
+        self._c(self._L.gf_overrelaxation(self._ctx, ctypes.c_ulonglong(seed), ctypes.c_ulonglong(sweep)))
+
     def links_from_algebra(self, omega):
         """U = exp(i sum_a omega_a lambda_a / 2) per link (the configs' seeded start)."""
         self.set_cold()

@@ -80,6 +80,20 @@ int gf_gaussian_momenta(gf_ctx *ctx, double sigma, unsigned long long seed, unsi
  * Gram-Schmidt normalized, third row conj(row1 x row2) */
 int gf_hot_start(gf_ctx *ctx, unsigned long long seed);
 
+/* heatbath!(U, temps, beta; seed, sweep, ITERATION_MAX) (heatbath/heatbathmodule.jl:2119-2163):
+ * one sweep of the plaquette-action SU(3) heatbath, mu = 1..4, even then odd sites, each link by
+ * the Cabibbo-Marinari subgroups (1,2), (2,3), (1,3) with Kennedy-Pendleton SU(2) steps
+ * (heatbath/portable/kernels.jl:63-300) at weight exp(beta/3 Re tr(U V)), V the staple sum
+ * (loops_staple; autostaples/wilsonloops.jl:395-412).  Streams (seed, sweep), tags 3-5; a link
+ * with a subgroup not accepted within itmax tries is left unchanged and counted in *failures.
+ * All extents must be even. */
+int gf_heatbath(gf_ctx *ctx, double beta, unsigned long long seed, unsigned long long sweep, int itmax,
+                long long *failures);
+/* overrelaxation!(U, ...) (heatbathmodule.jl:2183-2276, SUN_overrelaxation_rng! :1295-1321): same
+ * sweep order, three SU(2) reflections per link (subgroup chosen from stream (seed, sweep), tag
+ * 6), keeping the action unchanged */
+int gf_overrelaxation(gf_ctx *ctx, unsigned long long seed, unsigned long long sweep);
+
 /* ---- measurements ---------------------------------------------------------------- */
 /* calculate_Plaquette (AbstractGaugefields.jl:2684-2699): sum_{x,mu<nu} Re tr P_munu.
  * measure_plaquette (API.jl:395-404) divides by 6*V*3. */

@@ -1091,3 +1091,240 @@ void gfo_hot_links(const int *dims, uint64_t seed, double *U) {
             st(U + 18 * ((long)mu * V + s), m);
         }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Heatbath and overrelaxation (Cabibbo-Marinari over SU(2) subgroups)       */
+/* ------------------------------------------------------------------------- */
+/* project_onto_SU2! (heatbath/portable/kernels.jl:12-23): S -> [[a, -conj b], [b, conj a]],
+ * a = (S11 + conj S22)/2, b = (S21 - conj S12)/2 (a multiple of an SU(2) matrix) */
+typedef struct {
+    cplx e[2][2];
+} m2;
+static inline m2 su2_project(m2 S) {
+    const cplx a = 0.5 * S.e[0][0] + 0.5 * conj(S.e[1][1]);
+    const cplx b = 0.5 * S.e[1][0] - 0.5 * conj(S.e[0][1]);
+    m2 r;
+    r.e[0][0] = a;
+    r.e[1][0] = b;
+    r.e[0][1] = -conj(b);
+    r.e[1][1] = conj(a);
+    return r;
+}
+static inline m2 mul2(m2 a, m2 b) {
+    m2 r;
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) r.e[i][j] = a.e[i][0] * b.e[0][j] + a.e[i][1] * b.e[1][j];
+    return r;
+}
+static const int SUBGROUP[3][2] = {{0, 1}, {1, 2}, {0, 2}}; /* (1,2), (2,3), (1,3) */
+
+/* u <- embed(K at rows/cols n, m) u */
+static m3 su2_left_mul(m2 K, int n, int m, m3 u) {
+    m3 r = u;
+    for (int j = 0; j < 3; j++) {
+        r.e[n][j] = K.e[0][0] * u.e[n][j] + K.e[0][1] * u.e[m][j];
+        r.e[m][j] = K.e[1][0] * u.e[n][j] + K.e[1][1] * u.e[m][j];
+    }
+    return r;
+}
+
+static m3 reunit_rows(m3 u) { /* Gram-Schmidt rows 0, 1; row 2 = conj(row0 x row1) */
+    double n0 = 0.0;
+    for (int j = 0; j < 3; j++) n0 += creal(u.e[0][j] * conj(u.e[0][j]));
+    n0 = 1.0 / sqrt(n0);
+    for (int j = 0; j < 3; j++) u.e[0][j] *= n0;
+    cplx d = 0.0;
+    for (int j = 0; j < 3; j++) d += u.e[1][j] * conj(u.e[0][j]);
+    for (int j = 0; j < 3; j++) u.e[1][j] -= d * u.e[0][j];
+    double n1 = 0.0;
+    for (int j = 0; j < 3; j++) n1 += creal(u.e[1][j] * conj(u.e[1][j]));
+    n1 = 1.0 / sqrt(n1);
+    for (int j = 0; j < 3; j++) u.e[1][j] *= n1;
+    for (int k = 0; k < 3; k++) {
+        const int a = (k + 1) % 3, b = (k + 2) % 3;
+        u.e[2][k] = conj(u.e[0][a] * u.e[1][b] - u.e[0][b] * u.e[1][a]);
+    }
+    return u;
+}
+
+/* Kennedy-Pendleton SU(2) heatbath (_su2_update_kp_core!, heatbath/portable/kernels.jl:63-150)
+ * for the projected staple S (2x2, a multiple of SU(2)) and coupling beta / NC:
+ * k = 2 (beta/3) rho; tries draw (R, R', R'', R''') from block `try`; after acceptance
+ * (phi, cos theta) come from block 0xFFFF.  Returns 1 and K on success. */
+static int kp_update(uint64_t seed, uint64_t sweep, long s, int mu, int tag, double beta, int itmax, m2 S,
+                     m2 *K) {
+    const double rho0 = creal(S.e[0][0] + S.e[1][1]) / 2, rho1 = -cimag(S.e[0][1] + S.e[1][0]) / 2;
+    const double rho2 = creal(S.e[1][0] - S.e[0][1]) / 2, rho3 = cimag(S.e[1][1] - S.e[0][0]) / 2;
+    const double rho = sqrt(rho0 * rho0 + rho1 * rho1 + rho2 * rho2 + rho3 * rho3);
+    const cplx det = S.e[0][0] * S.e[1][1] - S.e[0][1] * S.e[1][0];
+    m2 V0;
+    V0.e[0][0] = rho * S.e[1][1] / det;
+    V0.e[0][1] = -rho * S.e[0][1] / det;
+    V0.e[1][0] = -rho * S.e[1][0] / det;
+    V0.e[1][1] = rho * S.e[0][0] / det;
+    const double k = 2.0 * (beta / 3.0) * rho;
+    double delta = 0.0;
+    int ok = 0;
+    for (int t = 0; t < itmax && !ok; t++) {
+        uint32_t w[4];
+        link_block(seed, sweep, s, mu, tag, t, w);
+        const double X = -log(unit32(w[0])) / k, Xp = -log(unit32(w[1])) / k;
+        const double c = cos(2.0 * GFO_PI * unit32(w[2]));
+        delta = Xp + X * c * c;
+        const double r3 = unit32(w[3]);
+        ok = r3 * r3 <= 1.0 - 0.5 * delta;
+    }
+    if (!ok) return 0;
+    uint32_t w[4];
+    link_block(seed, sweep, s, mu, tag, 0xFFFF, w);
+    const double a1 = 1.0 - delta, rr = sqrt(1.0 - a1 * a1);
+    const double phi = 2.0 * GFO_PI * unit32(w[0]);
+    const double ct = 2.0 * (unit32(w[1]) - 0.5), st = sqrt(1.0 - ct * ct);
+    const double a2 = rr * cos(phi) * st, a3 = rr * sin(phi) * st, a4 = rr * ct;
+    m2 tmp;
+    tmp.e[0][0] = CMPLX(a1, a4);
+    tmp.e[0][1] = CMPLX(a3, a2);
+    tmp.e[1][0] = CMPLX(-a3, a2);
+    tmp.e[1][1] = CMPLX(a1, -a4);
+    m2 Un = mul2(tmp, V0);
+    const cplx al = 0.5 * Un.e[0][0] + 0.5 * conj(Un.e[1][1]);
+    const cplx be = 0.5 * Un.e[1][0] - 0.5 * conj(Un.e[0][1]);
+    const double d = creal(al * conj(al)) + creal(be * conj(be));
+    K->e[0][0] = al / d;
+    K->e[1][0] = be / d;
+    K->e[0][1] = -conj(be) / d;
+    K->e[1][1] = conj(al) / d;
+    return 1;
+}
+
+/* SU3update_matrix! (heatbath/portable/kernels.jl:199-300): subgroups (1,2), (2,3), (1,3),
+ * stream tags 3, 4, 5; returns 0 and leaves u unchanged if a subgroup is not accepted */
+static int su3_heatbath_link(uint64_t seed, uint64_t sweep, long s, int mu, double beta, int itmax, m3 *u,
+                             m3 V) {
+    m3 w = *u;
+    for (int g = 0; g < 3; g++) {
+        const int n = SUBGROUP[g][0], m = SUBGROUP[g][1];
+        const m3 UV = mul(w, V);
+        m2 S;
+        S.e[0][0] = UV.e[n][n];
+        S.e[0][1] = UV.e[n][m];
+        S.e[1][0] = UV.e[m][n];
+        S.e[1][1] = UV.e[m][m];
+        m2 K;
+        if (!kp_update(seed, sweep, s, mu, 3 + g, beta, itmax, su2_project(S), &K)) return 0;
+        w = su2_left_mul(K, n, m, w);
+    }
+    *u = reunit_rows(w);
+    return 1;
+}
+
+/* overrelaxation subgroup step (_overrelaxation_subgroup!, heatbathmodule.jl:1248-1280):
+ * w = P_SU2(sub(u V)), h = (w w)^dag with normalized columns, u <- embed(h) u */
+static m3 su3_or_subgroup(m3 u, m3 V, int n, int m) {
+    const m3 UV = mul(u, V);
+    m2 S;
+    S.e[0][0] = UV.e[n][n];
+    S.e[0][1] = UV.e[n][m];
+    S.e[1][0] = UV.e[m][n];
+    S.e[1][1] = UV.e[m][m];
+    const m2 w = su2_project(S);
+    m2 h;
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) h.e[i][j] = conj(w.e[0][i]) * conj(w.e[j][0]) + conj(w.e[1][i]) * conj(w.e[j][1]);
+    for (int c = 0; c < 2; c++) { /* Gram-Schmidt on the columns */
+        if (c == 1) {
+            const cplx p = conj(h.e[0][0]) * h.e[0][1] + conj(h.e[1][0]) * h.e[1][1];
+            h.e[0][1] -= p * h.e[0][0];
+            h.e[1][1] -= p * h.e[1][0];
+        }
+        const double nn = 1.0 / sqrt(creal(h.e[0][c] * conj(h.e[0][c])) + creal(h.e[1][c] * conj(h.e[1][c])));
+        h.e[0][c] *= nn;
+        h.e[1][c] *= nn;
+    }
+    return su2_left_mul(h, n, m, u);
+}
+
+/* SUN_overrelaxation_rng! (heatbathmodule.jl:1295-1321): NC = 3 steps; step i draws the
+ * subgroup from block i of stream tag 6: n = 1 + floor(2 w0 / 2^32), m = n + 1 +
+ * floor((3 - n) w1 / 2^32) (the reference's nonuniform choice: (1,2) 1/4, (1,3) 1/4,
+ * (2,3) 1/2); the link is re-projected afterwards. */
+static m3 su3_overrelax_link(uint64_t seed, uint64_t sweep, long s, int mu, m3 u, m3 V) {
+    for (int i = 0; i < 3; i++) {
+        uint32_t w[4];
+        link_block(seed, sweep, s, mu, 6, i, w);
+        const int n = (int)(((uint64_t)w[0] * 2u) >> 32);         /* 0 or 1 */
+        const int m = n + 1 + (int)(((uint64_t)w[1] * (uint32_t)(2 - n)) >> 32);
+        u = su3_or_subgroup(u, V, n, m);
+    }
+    return reunit_rows(u);
+}
+
+static int site_parity(const lat *L, long s) {
+    long r = s;
+    int p = 0;
+    for (int m = 0; m < 4; m++) {
+        p += (int)(r % L->n[m]);
+        r /= L->n[m];
+    }
+    return p & 1;
+}
+
+/* heatbath!(U, temps, beta; seed, sweep) (heatbathmodule.jl:2119-2163): for mu = 1..4, even
+ * then odd sites: V = staple sum (loops_staple, in-tree make_plaq_staple autostaples/
+ * wilsonloops.jl:395-412), SU(3) heatbath of every link of that colour.  mode 0 heatbath,
+ * 1 overrelaxation.  Returns the number of links left unchanged by a failed heatbath. */
+long gfo_heatbath_sweep(const int *dims, double *U, double beta, uint64_t seed, uint64_t sweep, int itmax,
+                        int mode) {
+    lat L = lat_make(dims);
+    long failures = 0;
+    m3 *newu = (m3 *)malloc(sizeof(m3) * L.V);
+    char *did = (char *)malloc(L.V);
+    for (int mu = 0; mu < 4; mu++)
+        for (int par = 0; par < 2; par++) {
+            long fail_here = 0;
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static) reduction(+ : fail_here)
+#endif
+            for (long s = 0; s < L.V; s++) {
+                did[s] = 0;
+                if (site_parity(&L, s) != par) continue;
+                const m3 V = dag(staple_full(&L, U, mu, s));
+                m3 u = LINK(U, &L, mu, s);
+                if (mode == 0) {
+                    if (su3_heatbath_link(seed, sweep, s, mu, beta, itmax, &u, V)) {
+                        newu[s] = u;
+                        did[s] = 1;
+                    } else {
+                        fail_here++;
+                    }
+                } else {
+                    newu[s] = su3_overrelax_link(seed, sweep, s, mu, u, V);
+                    did[s] = 1;
+                }
+            }
+            failures += fail_here;
+            for (long s = 0; s < L.V; s++)
+                if (did[s]) st(LINKP(U, &L, mu, s), newu[s]);
+        }
+    free(newu);
+    free(did);
+    lat_free(&L);
+    return failures;
+}
+
+/* Kennedy-Pendleton sampler alone: a0 of n accepted SU(2) elements at coupling k (for the
+ * distribution test p(a0) ~ sqrt(1 - a0^2) exp(k a0)); stream tag 7, site = sample index */
+void gfo_kp_samples(double k, long n, uint64_t seed, double *a0) {
+    for (long i = 0; i < n; i++) {
+        m2 S; /* rho = k * 3 / (2 beta) with beta = 3/2: S = rho * identity */
+        S.e[0][0] = k;
+        S.e[1][1] = k;
+        S.e[0][1] = S.e[1][0] = 0.0;
+        m2 K;
+        if (!kp_update(seed, 0, i, 0, 7, 1.5, 1000000, S, &K)) {
+            a0[i] = NAN;
+            continue;
+        }
+        a0[i] = creal(K.e[0][0]);
+    }
+}

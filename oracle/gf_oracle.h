@@ -70,6 +70,13 @@ void gfo_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 void gfo_gaussian_momenta(const int *dims, double sigma, uint64_t seed, uint64_t sweep, double *P);
 void gfo_hot_links(const int *dims, uint64_t seed, double *U);
 
+/* heatbath (mode 0) / overrelaxation (mode 1) sweep over mu and checkerboard colours;
+ * returns the number of heatbath links left unchanged (no acceptance within itmax tries) */
+long gfo_heatbath_sweep(const int *dims, double *U, double beta, uint64_t seed, uint64_t sweep, int itmax,
+                        int mode);
+/* Kennedy-Pendleton SU(2) samples: a0 of n elements at coupling k */
+void gfo_kp_samples(double k, long n, uint64_t seed, double *a0);
+
 /* integrator: 0 = QPQ, 1 = PQP, 2 = Omelyan 2MN (lambda argument) */
 void gfo_md_trajectory(const int *dims, double *U, double *P, double coeff, int nlev,
                        double rho, int integrator, double lambda, int steps, double tau,

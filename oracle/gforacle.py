@@ -56,6 +56,9 @@ def lib():
                                          ctypes.POINTER(ctypes.c_uint32)]),
             "gfo_gaussian_momenta": (None, [_I, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64, _D]),
             "gfo_hot_links": (None, [_I, ctypes.c_uint64, _D]),
+            "gfo_heatbath_sweep": (ctypes.c_long, [_I, _D, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64,
+                                                   ctypes.c_int, ctypes.c_int]),
+            "gfo_kp_samples": (None, [ctypes.c_double, ctypes.c_long, ctypes.c_uint64, _D]),
             "gfo_md_trajectory": (None, [_I, _D, _D, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, _D]),
         }
@@ -258,3 +261,16 @@ def hot_links(dims, seed):
     U = np.zeros((4, int(np.prod(dims)), 9), np.complex128)
     lib().gfo_hot_links(_dims(dims), int(seed), _p(U))
     return U
+
+
+def heatbath_sweep(dims, U, beta, seed, sweep, itmax=100000, overrelax=False):
+    """in place on U; returns the number of links a heatbath left unchanged"""
+    assert U.flags["C_CONTIGUOUS"] and U.dtype == np.complex128
+    return lib().gfo_heatbath_sweep(_dims(dims), _p(U), float(beta), int(seed), int(sweep), int(itmax),
+                                    1 if overrelax else 0)
+
+
+def kp_samples(k, n, seed):
+    a0 = np.zeros(int(n))
+    lib().gfo_kp_samples(float(k), int(n), int(seed), _p(a0))
+    return a0

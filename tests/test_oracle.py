@@ -476,3 +476,72 @@ def test_oracle_random_fields(oracle):
     M = lt.gen.links_to_matrices(U)
     assert np.abs(M @ np.conj(np.swapaxes(M, -1, -2)) - np.eye(3)).max() < 1e-14
     assert np.abs(np.linalg.det(M) - 1).max() < 1e-14
+
+
+# ---------------------------------------------------------------- heatbath, overrelaxation
+@pytest.mark.parametrize("k", [0.5, 2.0, 8.0])
+def test_kennedy_pendleton_distribution(oracle, k):
+    # accepted a0 follow p(a0) ~ sqrt(1 - a0^2) exp(k a0) on [-1, 1] (Kennedy & Pendleton,
+    # Phys. Lett. B 156 (1985) 393; _su2_update_kp_core!, heatbath/portable/kernels.jl:63-150)
+    from scipy.integrate import quad
+
+    a = oracle.kp_samples(k, 400000, 7)
+    assert not np.isnan(a).any() and a.min() >= -1 and a.max() <= 1
+    w = lambda x: np.sqrt(1 - x * x) * np.exp(k * x)  # noqa: E731
+    Z = quad(w, -1, 1)[0]
+    for p in (1, 2, 3):
+        mp = quad(lambda x: x ** p * w(x), -1, 1)[0] / Z
+        sig = np.sqrt(((a ** p) ** 2).mean() - (a ** p).mean() ** 2) / np.sqrt(a.size)
+        assert abs((a ** p).mean() - mp) < 5 * sig
+
+
+def test_heatbath_strong_coupling(oracle):
+    # SU(3) strong coupling: <P> = beta/18 + beta^2/216 + O(beta^3)
+    dims = (6, 6, 6, 6)
+    U = lt.gen.cold_links(dims)
+    beta, ps = 0.3, []
+    for sw in range(120):
+        assert oracle.heatbath_sweep(dims, U, beta, 11, sw) == 0
+        if sw >= 20:
+            ps.append(oracle.plaquette_sum(dims, U) / (18 * np.prod(dims)))
+    err = np.std(ps) / np.sqrt(len(ps))
+    assert abs(np.mean(ps) - (beta / 18 + beta ** 2 / 216)) < 4 * err + 2e-4  # + beta^3 term
+
+
+def test_heatbath_agrees_with_hmc(oracle):
+    # both sample exp(beta/3 sum Re tr P): average plaquettes at beta = 5.5 on 4^4
+    dims = (4, 4, 4, 4)
+    beta = 5.5
+    U = lt.gen.cold_links(dims)
+    hb = []
+    for sw in range(160):
+        oracle.heatbath_sweep(dims, U, beta, 3, sw)
+        if sw >= 40:
+            hb.append(oracle.plaquette_sum(dims, U) / (18 * 256))
+    rng = np.random.default_rng(5)
+    W = U.copy()
+    hmc = []
+    for tr in range(160):
+        P = rng.standard_normal((4, 256, 8))
+        W1, P1 = W.copy(), P.copy()
+        H = oracle.md_trajectory(dims, W1, P1, beta / 2, 10, 1.0)
+        if rng.random() < np.exp(-H[2]):
+            W = W1
+        if tr >= 40:
+            hmc.append(oracle.plaquette_sum(dims, W) / (18 * 256))
+    # integrated autocorrelation ~ a few sweeps / trajectories: widen the error by 3
+    e = 3 * np.hypot(np.std(hb) / np.sqrt(len(hb)), np.std(hmc) / np.sqrt(len(hmc)))
+    assert abs(np.mean(hb) - np.mean(hmc)) < 4 * e
+
+
+def test_overrelaxation_keeps_action(oracle):
+    # overrelaxation is microcanonical: every link update keeps Re tr(U V)
+    dims = (4, 4, 4, 4)
+    U = alg_links(oracle, dims, 0.5, 41)
+    p0 = oracle.plaquette_sum(dims, U)
+    for sw in range(3):
+        oracle.heatbath_sweep(dims, U, 6.0, 5, sw, overrelax=True)
+    assert oracle.plaquette_sum(dims, U) == pytest.approx(p0, rel=1e-13)
+    M = lt.gen.links_to_matrices(U)
+    assert np.abs(M @ np.conj(np.swapaxes(M, -1, -2)) - np.eye(3)).max() < 1e-14
+    assert np.abs(U - alg_links(oracle, dims, 0.5, 41)).max() > 0.1  # the links did move
