@@ -661,12 +661,10 @@ class Heatbath:
     def __init__(self, U, beta, *, ITERATION_MAX=10 ** 5, seed=0, sweep=0, overrelaxation_sweep=None):
         if ITERATION_MAX <= 0:
             raise ArgumentError("ITERATION_MAX must be positive")
-        if ITERATION_MAX >= 0xFFFF:
-            raise ArgumentError("ITERATION_MAX must be below 65535 on the HIP backend (one counter block per try)")
         if any(int(d) % 2 for d in U.dims):
             raise ArgumentError(f"the checkerboard heatbath needs even lattice extents; got {list(U.dims)}")
         self.beta = float(beta)
-        self.ITERATION_MAX = int(ITERATION_MAX)
+        self.ITERATION_MAX = int(min(ITERATION_MAX, 2 ** 31 - 1))
         self.seed = int(seed)
         self.sweep = int(sweep)
         self.overrelaxation_sweep = int(sweep if overrelaxation_sweep is None else overrelaxation_sweep)

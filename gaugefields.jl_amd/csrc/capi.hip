@@ -7,6 +7,7 @@
 #include <type_traits>
 #include <algorithm>
 #include <utility>
+#include <vector>
 
 #include "../../include/gfhip.h"
 #include "internal.hpp"
@@ -25,7 +26,7 @@ struct gf_ctx {
     int device = 0;
     size_t esz = 8;  // bytes per real
     hipStream_t stream = nullptr;
-    // link-shaped fields (72 reals/site) and momentum-shaped fields (32 reals/site)
+    // SU(3) link fields (rows 0-1: 48 reals/site) and momentum-shaped fields (32 reals/site)
     void* U = nullptr;     // current links
     void* Ualt = nullptr;  // ping-pong partner of U
     void* Ubak = nullptr;  // Metropolis backup
@@ -183,7 +184,7 @@ int stout_alloc(gf_ctx* c) {
     }
     TRY(alloc(c, &c->D, mat_bytes(c)));
     TRY(alloc(c, &c->D2, mat_bytes(c)));
-    TRY(alloc(c, &c->G, mat_bytes(c)));
+    TRY(alloc(c, &c->G, mom_bytes(c)));  // Lam coefficients of the stout backward
     return GF_OK;
 }
 const void* level_in(gf_ctx* c, int l) { return l == 0 ? c->U : c->lev[l - 1]; }
@@ -327,6 +328,31 @@ int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
             c->geo.mbits[0] = -1;
             c->geo.mbits[1] = c->geo.mbits[2] = 0;
         }
+        // t-sweep tiles: 8 tiles of cy x cz (xy-group, z) columns, cz preferably 16
+        c->geo.tile[0] = c->geo.tile[1] = 0;
+        if (nxy % 64 == 0) {
+            const long long ngy = nxy / 64, nz = dims[2];
+            const int prefs[] = {16, 8, 32, 24, 12, 4, 48, 64, 6, 2, 3, 1};
+            for (int cz : prefs) {
+                if (cz > nz || nz % cz || (ngy * nz) % (8LL * cz)) continue;
+                const long long cy = ngy * nz / (8LL * cz);
+                if (cy < 1 || ngy % cy) continue;
+                c->geo.tile[0] = (int)cy;
+                c->geo.tile[1] = cz;
+                break;
+            }
+            if (c->geo.tile[0] > 0) {  // the block -> group map must be a permutation
+                const unsigned nb = (unsigned)(V / 64);
+                std::vector<char> seen(nb, 0);
+                bool ok = (nb % 8u) == 0;
+                for (unsigned b = 0; ok && b < nb; b++) {
+                    const int q = tsweep_group(c->geo, b, nb);
+                    ok = q >= 0 && (unsigned)q < nb && !seen[q];
+                    if (ok) seen[q] = 1;
+                }
+                if (!ok) c->geo.tile[0] = c->geo.tile[1] = 0;
+            }
+        }
     }
     c->prec = precision;
     c->esz = precision == GF_F64 ? 8 : 4;
@@ -412,7 +438,8 @@ static double su3_row2_deviation(const double* ref, size_t nlinks) {
             const double cr = (re(0, a) * re(1, b) - im(0, a) * im(1, b)) - (re(0, b) * re(1, a) - im(0, b) * im(1, a));
             const double ci = (re(0, a) * im(1, b) + im(0, a) * re(1, b)) - (re(0, b) * im(1, a) + im(0, b) * re(1, a));
             const double d = std::fabs(re(2, k) - cr) + std::fabs(im(2, k) + ci);
-            if (!(d <= worst)) worst = d;  // also catches NaN
+            if (std::isnan(d)) return d;
+            if (d > worst) worst = d;
         }
     }
     return worst;
@@ -686,7 +713,9 @@ static int heatbath_sweep(gf_ctx* c, int mode, double beta, unsigned long long s
             return fail(c, GF_EINVAL, "the checkerboard heatbath needs even lattice extents; got %d in direction %d",
                         c->geo.n[m], m + 1);
     if (itmax <= 0) return fail(c, GF_EINVAL, "ITERATION_MAX must be positive");
-    if (itmax >= 0xFFFF) return fail(c, GF_EINVAL, "ITERATION_MAX must be below 65535 (one counter block per try)");
+    // the try index is a 16-bit counter field (block 0xFFFF is reserved for the draws after an
+    // acceptance): at most 65534 proposals per SU(2) step, far beyond any acceptance rate met
+    itmax = std::min(itmax, 0xFFFE);
     if (mode == 0 && !(beta > 0.0 && std::isfinite(beta))) return fail(c, GF_EINVAL, "beta must be positive; got %g", beta);
     TRY(dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
@@ -803,7 +832,7 @@ int gf_stout_config(gf_ctx* c, int nlev, double rho, int mode) {
             if (!c->lev[l]) need_levels += link_bytes(c);
             if (!c->Cst[l]) need_c += mat_bytes(c);
         }
-        need_levels += (c->D ? 0 : mat_bytes(c)) + (c->D2 ? 0 : mat_bytes(c)) + (c->G ? 0 : mat_bytes(c));
+        need_levels += (c->D ? 0 : mat_bytes(c)) + (c->D2 ? 0 : mat_bytes(c)) + (c->G ? 0 : mom_bytes(c));
         mode = (double)(need_levels + need_c) * 1.1 <= (double)free_b ? GF_STOUT_STORE : GF_STOUT_RECOMPUTE;
     }
     // release stored C of a previous store-mode configuration if no longer needed

@@ -90,7 +90,8 @@ hipError_t launch_flow_stage(const Geo& g, const T* W, T* Wout, T* A, double alp
 template <typename T>
 hipError_t launch_stout_forward(const Geo& g, const T* Uin, T* Uout, T* Cst, double rho, hipStream_t st);
 template <typename T>
-hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* G,
+// Lc: scratch momentum-layout field (the 8 su(3) coefficients of Lam per link)
+hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc,
                                  double rho, hipStream_t st);
 
 }  // namespace gf

@@ -33,8 +33,7 @@ __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restric
         for (int k = 0; k < 8; k++) c[k] = alpha * c[k];
     }
     stp<T, GF_ST_NT>(A, V, mu, s, c);
-    M3<T> wn = mul(expi_herm(herm_from_coeffs(c, T(1))), w);
-    if (kFloatBulk<T>) reunitarize(wn);
+    const M3<T> wn = exp_coeffs_times(c, T(1), w);
     stu<T, GF_ST_NT>(Wout, V, mu, s, wn);
 }
 
@@ -54,21 +53,21 @@ __global__ __launch_bounds__(kBlock) void k_stout_forward(Geo g, const T* __rest
     const unsigned V = g.V;
     const M3<T> C = scal(rho, dag(staple_sigma(g, Uin, V, s, x, mu)));
     const M3<T> u = ldu(Uin, V, mu, s);
-    const M3<T> E = exp_antiherm(ta_mat(mul_nd(C, u)));
-    stu<T, GF_ST_NT>(Uout, V, mu, s, mul(E, u));
+    stu<T, GF_ST_NT>(Uout, V, mu, s, exp_antiherm_times(ta_mat(mul_nd(C, u)), u));
     if (STORE) stm<T, GF_ST_NT>(Cst, V, mu, s, C);
 }
 
 // ---------------------------------------------------------------------------
 // K6a: stout backward, site-local part (backward_dSdUαUβρ_add!, stout_fast.jl:317-407):
-//   Din_mu = Dout_mu e^{Q_mu} + (Lam_mu C_mu)^dag,   Lam_mu = TA(CdexpQdQ(U_mu Dout_mu, Q_mu)),
-//   G_mu   = U_mu^dag Lam_mu  (= dS/dC_mu)
+//   Din_mu = Dout_mu e^{Q_mu} + (Lam_mu C_mu)^dag,   Lam_mu = TA(CdexpQdQ(U_mu Dout_mu, Q_mu))
+// Lam_mu (traceless anti-Hermitian) is stored as its 8 su(3) coefficients; the staple part
+// uses G_mu = U_mu^dag Lam_mu (= dS/dC_mu) without storing it.
 // STORED reads C_mu from the forward pass, otherwise it is recomputed from the staples.
 // ---------------------------------------------------------------------------
 template <typename T, bool STORED>
 __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __restrict__ Uin,
                                                              const T* __restrict__ Cst, const T* __restrict__ Dout,
-                                                             T* __restrict__ Din, T* __restrict__ G, T rho) {
+                                                             T* __restrict__ Din, T* __restrict__ Lc, T rho) {
     const SiteMu sm = site_mu_index(g);
     const int s = sm.s, mu = sm.mu;
     if (s >= g.V) return;
@@ -87,10 +86,14 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
         exp_dexp(ta_mat(mul_nd(C, u)), mul(u, ldm(Dout, V, mu, s)), E, L);
     }
     const M3<T> Lam = ta_mat(L);
-    // re-read U_mu and Dout_mu (L1/L2 hits) instead of keeping them live across exp_dexp
+    {
+        T c[8];
+        ta_coeffs(Lam, c);
+        stp<T, GF_ST_NT>(Lc, V, mu, s, c);
+    }
+    // re-read Dout_mu (L1/L2 hit) instead of keeping it live across exp_dexp
     int s1 = s;
     asm volatile("" : "+v"(s1));
-    stm<T, GF_ST_NT>(G, V, mu, s, mul_dn(ldu(Uin, V, mu, s1), Lam));
     M3<T> D = mul(ldm(Dout, V, mu, s1), E);
     acc_dag(D, mul(Lam, C));
     stm<T, GF_ST_NT>(Din, V, mu, s, D);
@@ -98,65 +101,88 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
 
 // ---------------------------------------------------------------------------
 // K6b: stout backward, staple-derivative part (calc_dSdUν_fromdSCμ_add!, stout_fast.jl:712-785):
-// for output direction a at site y and each plane b != a,
+// for output direction a at site y and each plane b != a, with G = U^dag Lam,
 //   Din_a(y) += rho [ U_b(y+a) (G_a(y+b) U_b(y)^dag + U_a(y+b)^dag G_b(y))
 //                   + U_b(y-b+a)^dag (G_a(y-b) U_b(y-b) + U_a(y-b)^dag G_b(y-b)^dag)
 //                   + G_b(y+a)^dag U_a(y+b)^dag U_b(y)^dag
 //                   + G_b(y+a-b) U_a(y-b)^dag U_b(y-b) ]
+// evaluated with Lam directly (Lam^dag = -Lam): with X = U_b(y+a) U_a(y+b)^dag, S = X U_b(y)^dag,
+// Z = U_a(y-b)^dag U_b(y-b),
+//   upper = X Lam_a(y+b) U_b(y)^dag + S Lam_b(y) - Lam_b(y+a) S
+//   lower = U_b(y+a-b)^dag [U_a(y-b)^dag (Lam_a(y-b) - Lam_b(y-b)) U_b(y-b) + Lam_b(y+a-b) Z]
+// Six links and six 8-real Lam per plane instead of ten links and six 18-real G.
 // ---------------------------------------------------------------------------
 template <typename T>
+__device__ __forceinline__ M3<T> lam_of(const T* __restrict__ Lc, unsigned V, int mu, unsigned s) {
+    T c[8];
+    ldp(Lc, V, mu, s, c);
+    const M3<T> H = herm_from_coeffs(c, T(1));  // sum_a c_a lambda_a / 2
+    M3<T> A;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {  // i H
+        A.m[k].r = -H.m[k].i;
+        A.m[k].i = H.m[k].r;
+    }
+    return A;
+}
+
+template <typename T>
 __global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* __restrict__ U,
-                                                               const T* __restrict__ G, T* __restrict__ Din, T rho) {
+                                                               const T* __restrict__ Lc, T* __restrict__ Din, T rho) {
     const SiteMu sm = site_mu_index(g);
     const int y = sm.s, a = sm.mu;
     if (y >= g.V) return;
     int x[4];
     decode(g, y, x);
     const unsigned V = g.V;
-    {
-        M3<T> acc = zero3<T>();
-        const int ypa = y + dlt(g, x, a, 1);
+    M3<T> acc = zero3<T>();
+    const int ypa = y + dlt(g, x, a, 1);
+#pragma unroll 1
+    for (int b = 0; b < 4; b++) {
+        if (b == a) continue;
+        const int ypb0 = y + dlt(g, x, b, 1);
+        const int dmb = dlt(g, x, b, -1);
+        const int ymb0 = y + dmb, ypamb0 = ypa + dmb;
+        // each group loads its links through its own opaque index copies, so the compiler
+        // issues them group by group instead of hoisting all of them
+        {  // upper
+            int y0 = y, y1 = ypa, ypb = ypb0;
+            asm volatile("" : "+v"(y0), "+v"(y1), "+v"(ypb));
+            const M3<T> X = mul_nd(ldu(U, V, b, y1), ldu(U, V, a, ypb));
+            const M3<T> ub = ldu(U, V, b, y0);
+            acc_mul_nd(acc, mul(X, lam_of(Lc, V, a, ypb)), ub);
+            const M3<T> S = mul_nd(X, ub);
+            acc_mul(acc, S, lam_of(Lc, V, b, y0));
+            const M3<T> t = mul(lam_of(Lc, V, b, y1), S);
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            if (b == a) continue;
-            const int ypb0 = y + dlt(g, x, b, 1);
-            const int dmb = dlt(g, x, b, -1);
-            const int ymb0 = y + dmb, ypamb0 = ypa + dmb;
-            // each term group reloads its links through its own opaque index copies, so the
-            // compiler issues them group by group instead of hoisting all twelve link loads
-            {  // U_b(y+a) [G_a(y+b) U_b(y)^dag + U_a(y+b)^dag G_b(y)]
-                int y0 = y, y1 = ypa, ypb = ypb0;
-                asm volatile("" : "+v"(y0), "+v"(y1), "+v"(ypb));
-                M3<T> t = mul_nd(ldm(G, V, a, ypb), ldu(U, V, b, y0));
-                acc_mul_dn(t, ldu(U, V, a, ypb), ldm(G, V, b, y0));
-                acc_mul(acc, ldu(U, V, b, y1), t);
-            }
-            {  // U_b(y-b+a)^dag [G_a(y-b) U_b(y-b) + (G_b(y-b) U_a(y-b))^dag]
-                int ymb = ymb0, ypamb = ypamb0;
-                asm volatile("" : "+v"(ymb), "+v"(ypamb));
-                M3<T> t = mul(ldm(G, V, a, ymb), ldu(U, V, b, ymb));
-                acc_dag(t, mul(ldm(G, V, b, ymb), ldu(U, V, a, ymb)));
-                acc_mul_dn(acc, ldu(U, V, b, ypamb), t);
-            }
-            {  // (U_b(y) U_a(y+b) G_b(y+a))^dag
-                int y0 = y, y1 = ypa, ypb = ypb0;
-                asm volatile("" : "+v"(y0), "+v"(y1), "+v"(ypb));
-                acc_dag(acc, mul(mul(ldu(U, V, b, y0), ldu(U, V, a, ypb)), ldm(G, V, b, y1)));
-            }
-            {  // G_b(y+a-b) U_a(y-b)^dag U_b(y-b)
-                int ymb = ymb0, ypamb = ypamb0;
-                asm volatile("" : "+v"(ymb), "+v"(ypamb));
-                acc_mul(acc, ldm(G, V, b, ypamb), mul_dn(ldu(U, V, a, ymb), ldu(U, V, b, ymb)));
+            for (int k = 0; k < 9; k++) {
+                acc.m[k].r -= t.m[k].r;
+                acc.m[k].i -= t.m[k].i;
             }
         }
-        M3<T> d = ldm(Din, V, a, y);
+        {  // lower
+            int ymb = ymb0, ypamb = ypamb0;
+            asm volatile("" : "+v"(ymb), "+v"(ypamb));
+            const M3<T> ua = ldu(U, V, a, ymb), ub = ldu(U, V, b, ymb);
+            M3<T> d = lam_of(Lc, V, a, ymb);
+            const M3<T> lb = lam_of(Lc, V, b, ymb);
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            d.m[k].r = fma(rho, acc.m[k].r, d.m[k].r);
-            d.m[k].i = fma(rho, acc.m[k].i, d.m[k].i);
+            for (int k = 0; k < 9; k++) {
+                d.m[k].r -= lb.m[k].r;
+                d.m[k].i -= lb.m[k].i;
+            }
+            M3<T> t = mul(mul_dn(ua, d), ub);
+            acc_mul(t, lam_of(Lc, V, b, ypamb), mul_dn(ua, ub));
+            acc_mul_dn(acc, ldu(U, V, b, ypamb), t);
         }
-        stm<T, GF_ST_NT>(Din, V, a, y, d);
     }
+    M3<T> dn = ldm(Din, V, a, y);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        dn.m[k].r = fma(rho, acc.m[k].r, dn.m[k].r);
+        dn.m[k].i = fma(rho, acc.m[k].i, dn.m[k].i);
+    }
+    stm<T, GF_ST_NT>(Din, V, a, y, dn);
 }
 
 // ---------------------------------------------------------------------------
@@ -174,15 +200,15 @@ hipError_t launch_stout_forward(const Geo& g, const T* Uin, T* Uout, T* Cst, dou
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* G, double rho,
+hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc, double rho,
                                  hipStream_t st) {
     if (Cst)
-        k_stout_back_local<T, true><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Cst, Dout, Din, G, (T)rho);
+        k_stout_back_local<T, true><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Cst, Dout, Din, Lc, (T)rho);
     else
-        k_stout_back_local<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, G, (T)rho);
+        k_stout_back_local<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, Lc, (T)rho);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_stout_back_stencil<T><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, G, Din, (T)rho);
+    k_stout_back_stencil<T><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Lc, Din, (T)rho);
     return hipGetLastError();
 }
 

@@ -103,8 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U
         for (int a = 0; a < 8; a++) p[a] = fma(fac, c[a], p[a]);
         stp<T, GF_ST_NT>(FP, V, mu, s, p);
         if (MODE == FORCE_KICK_EXP) {
-            M3<T> un = mul(expi_herm(herm_from_coeffs(p, eps_u)), umu);
-            if (kFloatBulk<T>) reunitarize(un);
+            const M3<T> un = exp_coeffs_times(p, eps_u, umu);
             stu<T, GF_ST_NT>(Unew, V, mu, s, un);
         }
     }
@@ -122,8 +121,7 @@ __global__ __launch_bounds__(kBlock) void k_exp_update(Geo g, const T* __restric
     for (int mu = 0; mu < 4; mu++) {
         T p[8];
         ldp(P, V, mu, s, p);
-        M3<T> un = mul(expi_herm(herm_from_coeffs(p, eps)), ldu(Uin, V, mu, s));
-        if (kFloatBulk<T>) reunitarize(un);
+        const M3<T> un = exp_coeffs_times(p, eps, ldu(Uin, V, mu, s));
         stu<T, GF_ST_NT>(Uout, V, mu, s, un);
     }
 }

@@ -26,6 +26,9 @@ struct Geo {
     // Morton bit widths of the 64-site group coordinates (xy-group, z, t); mbits[0] < 0
     // selects plain linear group order (extents that are not powers of two)
     int mbits[3];
+    // t-sweep tiles (used when tile[0] > 0): the (xy-group, z) plane is cut into 8 tiles of
+    // tile[0] x tile[1] columns, one per XCD, and each XCD walks its columns slice by slice in t
+    int tile[2];
 };
 
 __device__ __forceinline__ void decode(const Geo& g, int s, int x[4]) {
@@ -60,11 +63,24 @@ __device__ __forceinline__ int site_index() { return (int)(xcd_block() * blockDi
 // Thread <-> (site, direction) map of the stencil kernels.  A workgroup of 256 threads is
 // 4 wavefronts over the same group of 64 consecutive sites; wavefront w handles link
 // direction mu = w.  The four directions of a site therefore run on one CU at the same
-// time and share their neighbour links through L1/L2.  Groups are visited in 4D Morton
-// order over (xy-group, z, t) so that the groups in flight on one XCD form a compact 4D
-// region whose halo stays in that XCD's L2.
+// time and share their neighbour links through L1/L2.  Group order: when the extents allow
+// it, each XCD owns one of 8 tiles of (xy-group, z) columns and sweeps it in t, so the slices
+// in flight re-read the previous slice from L2/MALL; otherwise 4D Morton order over
+// (xy-group, z, t) (power-of-two extents) or linear order.
+__host__ __device__ __forceinline__ int tsweep_group(const Geo& g, unsigned b, unsigned nb) {
+    const int ngy = g.n[0] * g.n[1] / 64;
+    const unsigned per = nb / 8u, xcd = b / per, loc = b % per;
+    const int nty = ngy / g.tile[0];
+    const int cols = g.tile[0] * g.tile[1];
+    const int t = (int)loc / cols, c = (int)loc % cols;
+    const int gy = (int)(xcd % (unsigned)nty) * g.tile[0] + c % g.tile[0];
+    const int z = (int)(xcd / (unsigned)nty) * g.tile[1] + c / g.tile[0];
+    return gy + ngy * (z + g.n[2] * t);
+}
+
 __device__ __forceinline__ int group_of_block(const Geo& g) {
     const unsigned b = xcd_block();
+    if (g.tile[0] > 0) return tsweep_group(g, b, gridDim.x);
     if (g.mbits[0] < 0) return (int)b;
     unsigned v0 = 0, v1 = 0, v2 = 0, bit = 0;
     const int top = max(g.mbits[0], max(g.mbits[1], g.mbits[2]));

@@ -6,6 +6,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace gf {
 
 template <typename T>
@@ -664,6 +666,50 @@ __device__ __forceinline__ void exp_dexp(const M3<T>& Q, const M3<T>& X, M3<T>& 
                f1i * X.m[k].r + f2r * HC.m[k].i + f2i * HC.m[k].r;
         L.m[k].r = im;
         L.m[k].i = -r;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// SU(3) link updates U' = exp(X) U.  For float storage the exponential, the product and the
+// projection back onto SU(3) all run in double and only the stored result is rounded, so each
+// update adds one rounding of the link instead of the accumulated float error of the product.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ M3<T> round_link(const M3<double>& d) {
+    M3<T> r;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        r.m[k].r = (T)d.m[k].r;
+        r.m[k].i = (T)d.m[k].i;
+    }
+    return r;
+}
+
+// exp(i t sum_a c_a lambda_a / 2) U
+template <typename T>
+__device__ __forceinline__ M3<T> exp_coeffs_times(const T c[8], T t, const M3<T>& u) {
+    if constexpr (std::is_same<T, float>::value) {
+        double cd[8];
+#pragma unroll
+        for (int a = 0; a < 8; a++) cd[a] = (double)c[a];
+        M3<double> r = mul(expi_herm(herm_from_coeffs(cd, (double)t)), to_f64(u));
+        reunitarize_f64(r);
+        return round_link<T>(r);
+    } else {
+        return mul(expi_herm(herm_from_coeffs(c, t)), u);
+    }
+}
+
+// exp(Q) U for a traceless anti-Hermitian Q
+template <typename T>
+__device__ __forceinline__ M3<T> exp_antiherm_times(const M3<T>& Q, const M3<T>& u) {
+    if constexpr (std::is_same<T, float>::value) {
+        M3<double> r = mul(exp_antiherm(to_f64(Q)), to_f64(u));
+        reunitarize_f64(r);
+        return round_link<T>(r);
+    } else {
+        return mul(exp_antiherm(Q), u);
     }
 }
 

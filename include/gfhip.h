@@ -85,8 +85,9 @@ int gf_hot_start(gf_ctx *ctx, unsigned long long seed);
  * the Cabibbo-Marinari subgroups (1,2), (2,3), (1,3) with Kennedy-Pendleton SU(2) steps
  * (heatbath/portable/kernels.jl:63-300) at weight exp(beta/3 Re tr(U V)), V the staple sum
  * (loops_staple; autostaples/wilsonloops.jl:395-412).  Streams (seed, sweep), tags 3-5; a link
- * with a subgroup not accepted within itmax tries is left unchanged and counted in *failures.
- * All extents must be even. */
+ * with a subgroup not accepted within itmax tries is left unchanged and counted in *failures
+ * (the try index is a 16-bit counter field: itmax above 65534 acts as 65534).  All extents
+ * must be even. */
 int gf_heatbath(gf_ctx *ctx, double beta, unsigned long long seed, unsigned long long sweep, int itmax,
                 long long *failures);
 /* overrelaxation!(U, ...) (heatbathmodule.jl:2183-2276, SUN_overrelaxation_rng! :1295-1321): same

@@ -1275,6 +1275,7 @@ static int site_parity(const lat *L, long s) {
  * 1 overrelaxation.  Returns the number of links left unchanged by a failed heatbath. */
 long gfo_heatbath_sweep(const int *dims, double *U, double beta, uint64_t seed, uint64_t sweep, int itmax,
                         int mode) {
+    if (itmax > 0xFFFE) itmax = 0xFFFE; /* 16-bit try index; block 0xFFFF follows an acceptance */
     lat L = lat_make(dims);
     long failures = 0;
     m3 *newu = (m3 *)malloc(sizeof(m3) * L.V);

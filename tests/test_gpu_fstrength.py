@@ -2,9 +2,9 @@
 energy density and the flow with measurements, through the C ABI, against the oracle.
 
 FP64: densities to absolute 1e-13 per site, sums to relative 1e-12 (absolute 1e-12 where
-the charge itself is near zero).  FP32 bulk: links are stored in float, the loop products and
-sums run in double, so the comparison is against the oracle on the same float-rounded links
-to 1e-12, and against the FP64 links to 1e-5 relative.
+the charge itself is near zero).  FP32 bulk: links are stored in float (rows 0 and 1; row 2
+rebuilt in float), the loop products and sums run in double, so the comparison is against the
+oracle on the float-rounded links to 1e-6 relative, and against the FP64 links to 1e-5.
 """
 import numpy as np
 import pytest
@@ -87,12 +87,13 @@ def test_fp32_bulk_topological_charge(Lat, oracle):
     U32 = U.astype(np.complex64).astype(np.complex128)
     L = Lat(dims, "f32")
     L.upload_links(U)
+    # the float links keep rows 0 and 1 and rebuild row 2 in float: ~1e-7 relative per link
     for m in METHODS:
         Qr32, qr32 = oracle.topological_charge_density(dims, U32, m)
-        assert np.abs(L.topological_charge_density(m) - qr32).max() < 1e-12, m
+        assert np.abs(L.topological_charge_density(m) - qr32).max() < 1e-6 * max(np.abs(qr32).max(), 1e-3), m
         Qr64 = oracle.topological_charge_density(dims, U, m)[0]
         assert abs(L.topological_charge(m) - Qr64) < 1e-5 * max(abs(Qr64), 1.0), m
-    assert L.energy_density_clover() == pytest.approx(oracle.energy_density_clover(dims, U32), rel=1e-12)
+    assert L.energy_density_clover() == pytest.approx(oracle.energy_density_clover(dims, U32), rel=1e-6)
 
 
 def test_flow_measure_matches_oracle(Lat, oracle):

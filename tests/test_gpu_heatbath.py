@@ -1,8 +1,10 @@
 """GPU heatbath / overrelaxation sweeps against the oracle on the same Philox streams.
 
 Same streams and the same floating-point recipe on both sides: the links after a sweep agree
-to rounding (FP64: 1e-12 absolute after two sweeps; an accept/reject decision flipping on a
-last-bit difference would show up as an O(1) link difference, and the test would catch it).
+to rounding amplified by the Kennedy-Pendleton map (a1 = 1 - delta near +-1 enters through
+sqrt(1 - a1^2), which multiplies last-bit differences of log/cos between the device and host
+math libraries by up to ~1e5): 1e-9 absolute after two sweeps.  An accept/reject decision
+flipping on a last-bit difference would show up as an O(1) link difference.
 Physics: strong-coupling plaquette, and overrelaxation keeps the action.
 """
 import numpy as np
@@ -40,8 +42,8 @@ def test_heatbath_matches_oracle(Lat, oracle, dims, beta):
         f = L.heatbath(beta, 99, sw)
         fo = oracle.heatbath_sweep(dims, W, beta, 99, sw)
         assert f == fo == 0
-    assert np.abs(L.download_links() - W).max() < 1e-12
-    assert L.plaquette_sum() == pytest.approx(oracle.plaquette_sum(dims, W), rel=1e-12)
+    assert np.abs(L.download_links() - W).max() < 1e-9
+    assert L.plaquette_sum() == pytest.approx(oracle.plaquette_sum(dims, W), rel=1e-11)
 
 
 def test_overrelaxation_matches_oracle_and_keeps_action(Lat, oracle):

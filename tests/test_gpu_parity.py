@@ -277,12 +277,14 @@ def test_fp32_bulk_against_fp64_oracle(Lat, oracle):
     assert abs(H[2] - Href[2]) < 1e-3
 
 
-# ------------------------------------------------------------------ Morton-ordered traversal
-# Extents whose xy-plane holds a multiple of 64 sites and whose extents are powers of two run
-# the stencil kernels in 4D Morton group order; these cases pin that path.
+# ------------------------------------------------------------------ workgroup orderings
+# Extents whose xy-plane holds a multiple of 64 sites run the stencil kernels in a non-linear
+# group order: the t-sweep tiles when the (xy-group, z) plane splits into 8 tiles ((8,8,8,8),
+# (16,8,4,8), (24,8,8,2)), else 4D Morton order for power-of-two extents ((32,4,2,4),
+# (8,8,2,2)); these cases pin both paths.
 
 
-@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (16, 8, 4, 8), (32, 4, 2, 4)])
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (16, 8, 4, 8), (24, 8, 8, 2), (32, 4, 2, 4), (8, 8, 2, 2)])
 def test_morton_path_md_and_flow(Lat, oracle, dims):
     L = Lat(dims)
     U = alg_links(oracle, dims, 0.3, 61)
