@@ -188,6 +188,64 @@ def stout_hmc_thermalized(gf, dims=DIMS, ntherm=6):
     }
 
 
+def production_stout_hmc(gf, dims=(48, 48, 48, 48), ntraj=50, check_every=10, seed=4848):
+    """Production-size stout HMC: SU(3) 48^4, beta 6, 3 stout levels at rho 0.12, Omelyan 2MN
+    16 steps, tau 1; FP32 links, momenta and forces (FP64 action/dH sums, Metropolis, CH
+    coefficients); start U = exp(i omega.T), omega ~ N(0, 0.3^2); momenta refreshed on the device
+    (Philox stream (seed, trajectory)).  Every check_every-th trajectory is rerun in FP64 from the
+    same links and momenta (dH error); the last trajectory is run back with tau -> -tau from its
+    end point (reversibility |U(tau -> -tau) - U0|)."""
+    gen = gf.generators
+    t_all = time.perf_counter()
+    L32 = gf.Lattice(dims, precision="f32")
+    L32.links_from_algebra(gen.algebra_coefficients(dims, 0.3, seed))
+    L32.stout_config(3, 0.12, "auto")
+    L64 = gf.Lattice(dims, precision="f64")
+    L64.stout_config(3, 0.12, "auto")
+    rng = np.random.default_rng(seed + 1)
+    times, dHs, acc, checks, plaq = [], [], 0, [], []
+    for k in range(ntraj):
+        L32.save_links()
+        L32.gaussian_momenta(seed, k)
+        check = check_every > 0 and k % check_every == 0
+        if check:
+            U0 = L32.download_links()
+            P0 = L32.download_momenta()
+        L32.sync()
+        t0 = time.perf_counter()
+        H = L32.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True)
+        L32.sync()
+        times.append(time.perf_counter() - t0)
+        dHs.append(H[2])
+        if check:
+            L64.upload_links(U0)
+            L64.upload_momenta(P0)
+            H64 = L64.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True)
+            checks.append({"trajectory": k, "dH_fp32": H[2], "dH_fp64": H64[2], "abs_diff": abs(H[2] - H64[2])})
+        if rng.random() < np.exp(min(0.0, -H[2])):
+            acc += 1
+        else:
+            L32.restore_links()
+        plaq.append(L32.plaquette())
+        log(f"[production] traj {k}: dH {H[2]:.4g}, {times[-1]:.3f} s, plaquette {plaq[-1]:.6f}")
+    # reversibility of the last trajectory from the current links
+    U0 = L32.download_links()
+    L32.gaussian_momenta(seed, ntraj)
+    L32.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True, diagnostics=False)
+    L32.md_trajectory(COEFF, 16, -1.0, "2MN", fused=True, diagnostics=False)
+    rev = float(np.abs(L32.download_links() - U0).max())
+    mode = L32.stout_mode()
+    L32.close()
+    L64.close()
+    return {"lattice": "x".join(map(str, dims)), "sites": int(np.prod(dims)), "precision": "fp32 bulk",
+            "levels": 3, "rho": 0.12, "integrator": "2MN", "steps": 16, "tau": 1.0, "trajectories": ntraj,
+            "stout_memory_mode": mode, "s_per_traj_mean": float(np.mean(times)),
+            "s_per_traj_min": float(np.min(times)), "acceptance": acc / ntraj, "dH": dHs,
+            "plaquette": plaq, "fp64_reruns": checks,
+            "max_abs_dH_diff_fp32_fp64": max(c["abs_diff"] for c in checks) if checks else None,
+            "reversibility_max_abs_dU": rev, "wall_s": time.perf_counter() - t_all}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +255,8 @@ def main():
     ap.add_argument("--no-stout", action="store_true", help="skip the stout-HMC s/traj lines")
     ap.add_argument("--stout-therm", type=int, default=6, help="FP64 thermalization trajectories")
     ap.add_argument("--stout-only", action="store_true", help="run only the stout-HMC and flow comparisons")
+    ap.add_argument("--production", type=int, default=0, metavar="NTRAJ",
+                    help="run only the 48^4 FP32-bulk stout HMC with NTRAJ trajectories")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -204,6 +264,9 @@ def main():
         sys.exit(2)
 
     gf = __graft_entry__.load_package()
+    if args.production:
+        print(json.dumps({"production_stout_hmc": production_stout_hmc(gf, ntraj=args.production)}))
+        return
     if args.stout_only:  # profiling aid: only the stout-HMC and flow comparisons
         print(json.dumps({"stout_hmc": stout_hmc_thermalized(gf, ntherm=args.stout_therm),
                           "wilson_flow": flow_compare(gf, steps=30)}))
