@@ -167,8 +167,9 @@ def gauge_configuration(lattice, *, backend=None, colors=3, halo=1, start="cold"
                         eltype=np.complex128, boundary="periodic", verbose=0, device=0):
     """Create a configuration on the GPU (gauge_configuration, API.jl:178-251).
 
-    start="cold": unit links.  start="hot": each link a seeded complex-Gaussian matrix
-    projected to SU(3) (see generators.hot_start_links); ``seed=None`` draws a fresh seed."""
+    start="cold": unit links.  start="hot": the hot fill of the legacy backend (entries
+    (u - 1/2) + i (u' - 1/2), rows Gram-Schmidt normalized to SU(3)) generated on the device from
+    the Philox4x32-10 stream of ``seed``; ``seed=None`` draws a fresh seed."""
     if backend is not None and not isinstance(backend, HIPBackend):
         raise ArgumentError("only HIPBackend() is available here")
     dims = tuple(int(d) for d in lattice)
@@ -191,7 +192,7 @@ def gauge_configuration(lattice, *, backend=None, colors=3, halo=1, start="cold"
     if start == "hot":
         if seed is None:
             seed = int(np.random.SeedSequence().entropy % (2 ** 63))
-        L.upload_links(generators.hot_start_links(dims, seed))
+        L.hot_start(int(seed) % (2 ** 64))
     return U
 
 
@@ -213,15 +214,16 @@ def gauge_momenta(U):
 
 
 def gaussian_momenta_(momenta, *, sigma=1.0, seed=None, sweep=0):
-    """Fill momenta with N(0, sigma^2) coefficients (gaussian_momenta!, API.jl:341-368).
-    The stream is numpy Philox(seed, sweep), not the reference's LatticeMatrices stream."""
+    """Fill momenta with N(0, sigma^2) coefficients (gaussian_momenta!, API.jl:341-368), generated
+    on the device from the Philox4x32-10 per-link stream (seed, sweep) -- not the reference's
+    LatticeMatrices stream."""
     if sweep < 0:
         raise ArgumentError(f"sweep must be nonnegative; got {sweep}")
+    if not sigma >= 0:
+        raise ArgumentError(f"sigma must be nonnegative; got {sigma}")
     if seed is None:
         seed = int(np.random.SeedSequence().entropy % (2 ** 63))
-    dims = momenta.U.dims
-    P = generators.gaussian_momenta(dims, int(seed) * 1000003 + int(sweep), sigma=sigma)
-    momenta.lattice.upload_momenta(P)
+    momenta.lattice.gaussian_momenta(int(seed) % (2 ** 64), int(sweep), sigma=float(sigma))
     return momenta
 
 
