@@ -19,8 +19,9 @@ __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restric
     int x[4];
     decode(g, s, x);
     const unsigned V = g.V;
-    const M3<T> sig = staple_sigma(g, W, V, s, x, mu);
-    const M3<T> w = ldu(W, V, mu, s);
+    R6<T> own;
+    const M3<T> sig = staple_sigma_own(g, W, V, s, x, mu, own);
+    const M3<T> w = full_of(own);
     T c[8];
     ta_coeffs(mul(w, sig), c);
     if (beta != T(0)) {
@@ -51,8 +52,9 @@ __global__ __launch_bounds__(kBlock) void k_stout_forward(Geo g, const T* __rest
     int x[4];
     decode(g, s, x);
     const unsigned V = g.V;
-    const M3<T> C = scal(rho, dag(staple_sigma(g, Uin, V, s, x, mu)));
-    const M3<T> u = ldu(Uin, V, mu, s);
+    R6<T> own;
+    const M3<T> C = scal(rho, dag(staple_sigma_own(g, Uin, V, s, x, mu, own)));
+    const M3<T> u = full_of(own);
     stu<T, GF_ST_NT>(Uout, V, mu, s, exp_antiherm_times(ta_mat(mul_nd(C, u)), u));
     if (STORE) stm<T, GF_ST_NT>(Cst, V, mu, s, C);
 }
@@ -73,16 +75,18 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
     if (s >= g.V) return;
     const unsigned V = g.V;
     M3<T> C;
+    R6<T> own;
     if (STORED) {
         C = ldm(Cst, V, mu, s);
-    } else {
+        own = ldr(Uin, V, mu, s);
+    } else {  // the same staple evaluation (and rounding) as the forward kernel
         int x[4];
         decode(g, s, x);
-        C = scal(rho, dag(staple_sigma(g, Uin, V, s, x, mu)));
+        C = scal(rho, dag(staple_sigma_own(g, Uin, V, s, x, mu, own)));
     }
     M3<T> E, L;
     {
-        const M3<T> u = ldu(Uin, V, mu, s);
+        const M3<T> u = full_of(own);
         exp_dexp(ta_mat(mul_nd(C, u)), mul(u, ldm(Dout, V, mu, s)), E, L);
     }
     const M3<T> Lam = ta_mat(L);

@@ -88,8 +88,9 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U
     int x[4];
     decode(g, s, x);
     const unsigned V = g.V;
-    const M3<T> sig = staple_sigma(g, U, V, s, x, mu);
-    const M3<T> umu = ldu(U, V, mu, s);
+    R6<T> own;
+    const M3<T> sig = staple_sigma_own(g, U, V, s, x, mu, own);
+    const M3<T> umu = full_of(own);
     T c[8];
     ta_coeffs(mul(umu, sig), c);
     if (MODE == FORCE_STORE) {

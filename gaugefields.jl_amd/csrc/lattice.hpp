@@ -153,19 +153,24 @@ __device__ __forceinline__ void buf_st(float2 v, __amdgpu_buffer_rsrc_t r, unsig
                                           (int)voff, (int)soff, AUX);
 }
 
-// SU(3) link field: planes (mu*6 + k), k = 3*i + j < 6, hold rows 0 and 1 of U_mu
+// SU(3) link field: planes (mu*6 + k), k = 3*i + j < 6, hold rows 0 and 1 of U_mu.
+// R6 is a link as loaded (its two stored rows); full_of() rebuilds the third row.
 template <typename T>
-__device__ __forceinline__ M3<T> ldu(const T* __restrict__ p, unsigned V, int mu, unsigned s) {
+struct R6 {
+    T2<T> v[6];
+};
+template <typename T>
+__device__ __forceinline__ R6<T> ldr(const T* __restrict__ p, unsigned V, int mu, unsigned s) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 6u * mu, 6u);
     const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
-    M3<T> m;
+    R6<T> o;
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        const T2<T> v = buf_ld(r, off, k * ps, (const T*)nullptr);
-        m.m[k].r = v.x;
-        m.m[k].i = v.y;
-    }
-    // row2_k = conj(row0_{k+1} row1_{k+2} - row0_{k+2} row1_{k+1})
+    for (int k = 0; k < 6; k++) o.v[k] = buf_ld(r, off, k * ps, (const T*)nullptr);
+    return o;
+}
+// row2_k = conj(row0_{k+1} row1_{k+2} - row0_{k+2} row1_{k+1}): exact for SU(3) (unitary, det 1)
+template <typename T>
+__device__ __forceinline__ void complete_row2(M3<T>& m) {
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const int a = (k + 1) % 3, b = (k + 2) % 3;
@@ -173,7 +178,61 @@ __device__ __forceinline__ M3<T> ldu(const T* __restrict__ p, unsigned V, int mu
         m.m[6 + k].r = (p0.r * q1.r - p0.i * q1.i) - (p1.r * q0.r - p1.i * q0.i);
         m.m[6 + k].i = -((p0.r * q1.i + p0.i * q1.r) - (p1.r * q0.i + p1.i * q0.r));
     }
+}
+template <typename T>
+__device__ __forceinline__ M3<T> full_of(const R6<T>& r) {
+    M3<T> m;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        m.m[k].r = r.v[k].x;
+        m.m[k].i = r.v[k].y;
+    }
+    complete_row2(m);
     return m;
+}
+template <typename T>
+__device__ __forceinline__ M3<T> ldu(const T* __restrict__ p, unsigned V, int mu, unsigned s) {
+    return full_of(ldr(p, V, mu, s));
+}
+template <typename T>
+__device__ __forceinline__ cx<T> rc(const R6<T>& r, int k) {
+    cx<T> c;
+    c.r = r.v[k].x;
+    c.i = r.v[k].y;
+    return c;
+}
+// products whose result is SU(3): rows 0 and 1 from the two stored rows of a, row 2 rebuilt
+template <typename T>
+__device__ __forceinline__ M3<T> su3_mul_nd(const R6<T>& a, const M3<T>& b) {  // a b^dag
+    M3<T> c;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cx<T> t;
+            t.r = t.i = T(0);
+#pragma unroll
+            for (int k = 0; k < 3; k++) cmac_nc(t, rc(a, 3 * i + k), b.m[3 * j + k]);
+            c.m[3 * i + j] = t;
+        }
+    complete_row2(c);
+    return c;
+}
+template <typename T>
+__device__ __forceinline__ M3<T> su3_mul(const R6<T>& a, const M3<T>& b) {  // a b
+    M3<T> c;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cx<T> t;
+            t.r = t.i = T(0);
+#pragma unroll
+            for (int k = 0; k < 3; k++) cmac(t, rc(a, 3 * i + k), b.m[3 * k + j]);
+            c.m[3 * i + j] = t;
+        }
+    complete_row2(c);
+    return c;
 }
 
 template <typename T, int AUX = GF_ST_DEFAULT>
@@ -250,7 +309,78 @@ __device__ __forceinline__ void stp(T* __restrict__ p, unsigned V, int mu, unsig
 // (staples of calc_dSdUμ!, GaugeActions.jl:95-123).  The same sum is V_mu^dag for the
 // loops_staple_prime staple V_mu of the flow (autostaples/wilsonloops.jl:468-485) and C_mu/rho
 // of stout smearing (stout_fast.jl:603-624).
+//
+// The 18 staple links (and, for staple_sigma_own, U_mu(x) itself as a 19th) are loaded one
+// link ahead of their use: the loads of link k+1 are issued before the arithmetic on link k,
+// and a scheduling barrier keeps the compiler from hoisting further loads (which would keep
+// dozens of matrices live) or sinking these to their use.  The two products of each staple
+// have SU(3) intermediates and are formed as su3_mul / su3_mul_nd.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int sel4(int i, int a0, int a1, int a2, int a3) {
+    return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
+}
+template <typename T, bool OWN>
+__device__ __forceinline__ M3<T> staple_sigma_pipe(const Geo& g, const T* __restrict__ U, unsigned V, int s,
+                                                   const int x[4], int mu, R6<T>* own) {
+    int fw[4], bw[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        fw[d] = dlt(g, x, d, 1);
+        bw[d] = dlt(g, x, d, -1);
+    }
+    const int fmu = sel4(mu, fw[0], fw[1], fw[2], fw[3]);
+    // link k < 18: block j = k / 6 of direction nu = j + (j >= mu), slot k % 6:
+    //   0 U_nu(x+mu)  1 U_mu(x+nu)  2 U_nu(x)  3 U_mu(x-nu)  4 U_nu(x+mu-nu)  5 U_nu(x-nu)
+    // link 18: U_mu(x)
+    auto issue = [&](int k) -> R6<T> {
+        if (k == 18) return ldr(U, V, mu, (unsigned)s);
+        const int j = k / 6, r = k % 6;
+        const int nu = j + (j >= mu ? 1 : 0);
+        const int fnu = sel4(nu, fw[0], fw[1], fw[2], fw[3]);
+        const int bnu = sel4(nu, bw[0], bw[1], bw[2], bw[3]);
+        int dir = nu, site = s + bnu;
+        if (r == 0) site = s + fmu;
+        if (r == 1) dir = mu, site = s + fnu;
+        if (r == 2) site = s;
+        if (r == 3) dir = mu;
+        if (r == 4) site = s + fmu + bnu;
+        return ldr(U, V, dir, (unsigned)site);
+    };
+    constexpr int NL = OWN ? 19 : 18;
+    M3<T> sig = zero3<T>();
+    M3<T> X;
+    R6<T> A, nxt = issue(0);
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const R6<T> cur = nxt;
+        if (k + 1 < NL) nxt = issue(k + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        if (k == 18) {
+            *own = cur;
+            break;
+        }
+        switch (k % 6) {
+            case 0: A = cur; break;                                  // U_nu(x+mu)
+            case 1: X = su3_mul_nd(A, full_of(cur)); break;          // U_nu(x+mu) U_mu(x+nu)^dag
+            case 2: acc_mul_nd(sig, X, full_of(cur)); break;         // ... U_nu(x)^dag
+            case 3: A = cur; break;                                  // U_mu(x-nu)
+            case 4: X = su3_mul(A, full_of(cur)); break;             // U_mu(x-nu) U_nu(x+mu-nu)
+            default: acc_mul_dn(sig, X, full_of(cur)); break;        // (...)^dag U_nu(x-nu)
+        }
+    }
+    return sig;
+}
+// the staple sum and U_mu(x) (own, as loaded): force, flow and stout-forward kernels
+template <typename T>
+__device__ __forceinline__ M3<T> staple_sigma_own(const Geo& g, const T* __restrict__ U, unsigned V, int s,
+                                                  const int x[4], int mu, R6<T>& own) {
+    return staple_sigma_pipe<T, true>(g, U, V, s, x, mu, &own);
+}
+
+// Unpipelined form for kernels with large register state of their own (heatbath, the stout
+// backward's recomputation, dS/dU), where the prefetch buffer would cost occupancy: links
+// are loaded at their use, through opaque copies of the site indices so that the compiler
+// neither merges them with the loads of other (mu, nu) iterations nor hoists them all.
 template <typename T>
 __device__ __forceinline__ M3<T> staple_sigma(const Geo& g, const T* __restrict__ U, unsigned V, int s,
                                               const int x[4], int mu) {
@@ -259,9 +389,6 @@ __device__ __forceinline__ M3<T> staple_sigma(const Geo& g, const T* __restrict_
 #pragma unroll
     for (int nu = 0; nu < 4; nu++) {
         if (nu == mu) continue;
-        // opaque copies of the site indices: stops the compiler from merging these link
-        // loads with identical ones of other (mu, nu) iterations (or hoisting them all
-        // ahead), which would keep dozens of 3x3 matrices live and spill
         {  // upper: (U_nu(x+mu) U_mu(x+nu)^dag) U_nu(x)^dag
             int s0 = s, s1 = spm, spn = s + dlt(g, x, nu, 1);
             asm volatile("" : "+v"(s0), "+v"(s1), "+v"(spn));
