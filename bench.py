@@ -360,7 +360,7 @@ def main():
         "site_updates_per_s": V / (ms_per_step * 1e-3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": recorded_traffic("void gf::k_force<double, 2>")},
+                     "traffic": recorded_traffic("void gf::k_force<double, 2, true>")},
         "measured_copy_GBps": copy_gbps,
         "kernels": kernels,
         "stout_hmc": stout,

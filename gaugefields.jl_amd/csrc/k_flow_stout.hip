@@ -10,17 +10,20 @@ namespace gf {
 //   Wout_mu = exp(A_mu) W_mu               (exp_aF_U!, AbstractGaugefields.jl:2810-2841)
 // stages (alpha, beta) = (-eps/4, 0), (-8eps/9, -17/9), (-3eps/4, -1)
 // ---------------------------------------------------------------------------
-template <typename T>
+// SHARED: upper-staple products shared between directions through LDS (V % 64 == 0)
+template <typename T, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restrict__ W, T* __restrict__ Wout,
                                                        T* __restrict__ A, T alpha, T beta) {
+    __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
     const SiteMu sm = site_mu_index(g);
-    const int s = sm.s, mu = sm.mu;
-    if (s >= g.V) return;
+    const int mu = sm.mu;
+    int s;
+    if (!stencil_site<SHARED>(g, sm, s)) return;
     int x[4];
     decode(g, s, x);
     const unsigned V = g.V;
     R6<T> own;
-    const M3<T> sig = staple_sigma_own(g, W, V, s, x, mu, own);
+    const M3<T> sig = staple_sum_own<T, SHARED>(g, W, V, s, x, mu, own, xs);
     const M3<T> w = full_of(own);
     T c[8];
     ta_coeffs(mul(w, sig), c);
@@ -43,17 +46,19 @@ __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restric
 //   C_mu = rho V_mu = rho Sigma_mu^dag,  Q_mu = TA(C_mu U_mu^dag),  Uout_mu = exp(Q_mu) U_mu
 // STORE keeps C_mu for the backward pass.
 // ---------------------------------------------------------------------------
-template <typename T, bool STORE>
+template <typename T, bool STORE, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_stout_forward(Geo g, const T* __restrict__ Uin, T* __restrict__ Uout,
                                                           T* __restrict__ Cst, T rho) {
+    __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
     const SiteMu sm = site_mu_index(g);
-    const int s = sm.s, mu = sm.mu;
-    if (s >= g.V) return;
+    const int mu = sm.mu;
+    int s;
+    if (!stencil_site<SHARED>(g, sm, s)) return;
     int x[4];
     decode(g, s, x);
     const unsigned V = g.V;
     R6<T> own;
-    const M3<T> C = scal(rho, dag(staple_sigma_own(g, Uin, V, s, x, mu, own)));
+    const M3<T> C = scal(rho, dag(staple_sum_own<T, SHARED>(g, Uin, V, s, x, mu, own, xs)));
     const M3<T> u = full_of(own);
     stu<T, GF_ST_NT>(Uout, V, mu, s, exp_antiherm_times(ta_mat(mul_nd(C, u)), u));
     if (STORE) stm<T, GF_ST_NT>(Cst, V, mu, s, C);
@@ -66,13 +71,15 @@ __global__ __launch_bounds__(kBlock) void k_stout_forward(Geo g, const T* __rest
 // uses G_mu = U_mu^dag Lam_mu (= dS/dC_mu) without storing it.
 // STORED reads C_mu from the forward pass, otherwise it is recomputed from the staples.
 // ---------------------------------------------------------------------------
-template <typename T, bool STORED>
+template <typename T, bool STORED, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __restrict__ Uin,
                                                              const T* __restrict__ Cst, const T* __restrict__ Dout,
                                                              T* __restrict__ Din, T* __restrict__ Lc, T rho) {
+    __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
     const SiteMu sm = site_mu_index(g);
-    const int s = sm.s, mu = sm.mu;
-    if (s >= g.V) return;
+    const int mu = sm.mu;
+    int s;
+    if (!stencil_site<SHARED>(g, sm, s)) return;
     const unsigned V = g.V;
     M3<T> C;
     R6<T> own;
@@ -82,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
     } else {  // the same staple evaluation (and rounding) as the forward kernel
         int x[4];
         decode(g, s, x);
-        C = scal(rho, dag(staple_sigma_own(g, Uin, V, s, x, mu, own)));
+        C = scal(rho, dag(staple_sum_own<T, SHARED>(g, Uin, V, s, x, mu, own, xs)));
     }
     M3<T> E, L;
     {
@@ -192,24 +199,36 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* _
 // ---------------------------------------------------------------------------
 template <typename T>
 hipError_t launch_flow_stage(const Geo& g, const T* W, T* Wout, T* A, double alpha, double beta, hipStream_t st) {
-    k_flow_stage<T><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, W, Wout, A, (T)alpha, (T)beta);
+    if (g.V % 64 == 0)
+        k_flow_stage<T, true><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, W, Wout, A, (T)alpha, (T)beta);
+    else
+        k_flow_stage<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, W, Wout, A, (T)alpha, (T)beta);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_stout_forward(const Geo& g, const T* Uin, T* Uout, T* Cst, double rho, hipStream_t st) {
-    if (Cst)
-        k_stout_forward<T, true><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Uout, Cst, (T)rho);
+    const int nb = grid_sites_mu(g.V);
+    const bool sh = g.V % 64 == 0;
+    if (Cst && sh)
+        k_stout_forward<T, true, true><<<nb, kBlock, 0, st>>>(g, Uin, Uout, Cst, (T)rho);
+    else if (Cst)
+        k_stout_forward<T, true, false><<<nb, kBlock, 0, st>>>(g, Uin, Uout, Cst, (T)rho);
+    else if (sh)
+        k_stout_forward<T, false, true><<<nb, kBlock, 0, st>>>(g, Uin, Uout, nullptr, (T)rho);
     else
-        k_stout_forward<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Uout, nullptr, (T)rho);
+        k_stout_forward<T, false, false><<<nb, kBlock, 0, st>>>(g, Uin, Uout, nullptr, (T)rho);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc, double rho,
                                  hipStream_t st) {
+    const int nb = grid_sites_mu(g.V);
     if (Cst)
-        k_stout_back_local<T, true><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Cst, Dout, Din, Lc, (T)rho);
+        k_stout_back_local<T, true, false><<<nb, kBlock, 0, st>>>(g, Uin, Cst, Dout, Din, Lc, (T)rho);
+    else if (g.V % 64 == 0)  // recompute C exactly as k_stout_forward<T, *, true> formed it
+        k_stout_back_local<T, false, true><<<nb, kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, Lc, (T)rho);
     else
-        k_stout_back_local<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, Lc, (T)rho);
+        k_stout_back_local<T, false, false><<<nb, kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, Lc, (T)rho);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     k_stout_back_stencil<T><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Lc, Din, (T)rho);

@@ -79,17 +79,20 @@ __global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U,
 //   FORCE_STORE    : FP = F
 //   FORCE_KICK     : FP += eps_p F                    (update_momenta!, :539-551)
 //   FORCE_KICK_EXP : FP += eps_p F; Unew = exp(eps_u FP) U   (then update_gaugefields!, :513-531)
-template <typename T, int MODE>
+// SHARED: upper-staple products shared between directions through LDS (V % 64 == 0)
+template <typename T, int MODE, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U, T* __restrict__ FP,
                                                   T* __restrict__ Unew, T fac, T eps_u) {
+    __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
     const SiteMu sm = site_mu_index(g);
-    const int s = sm.s, mu = sm.mu;
-    if (s >= g.V) return;
+    const int mu = sm.mu;
+    int s;
+    if (!stencil_site<SHARED>(g, sm, s)) return;
     int x[4];
     decode(g, s, x);
     const unsigned V = g.V;
     R6<T> own;
-    const M3<T> sig = staple_sigma_own(g, U, V, s, x, mu, own);
+    const M3<T> sig = staple_sum_own<T, SHARED>(g, U, V, s, x, mu, own, xs);
     const M3<T> umu = full_of(own);
     T c[8];
     ta_coeffs(mul(umu, sig), c);
@@ -267,15 +270,27 @@ hipError_t launch_force(int mode, const Geo& g, const T* U, T* FP, T* Unew, doub
                         double eps_u, hipStream_t st) {
     const int nb = grid_sites_mu(g.V);
     const double fac = -coeff / 3.0;
+    const bool sh = g.V % 64 == 0;
     switch (mode) {
         case FORCE_STORE:
-            k_force<T, FORCE_STORE><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)fac, (T)eps_u);
+            if (sh)
+                k_force<T, FORCE_STORE, true><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)fac, (T)eps_u);
+            else
+                k_force<T, FORCE_STORE, false><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)fac, (T)eps_u);
             break;
         case FORCE_KICK:
-            k_force<T, FORCE_KICK><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)(fac * eps_p), (T)eps_u);
+            if (sh)
+                k_force<T, FORCE_KICK, true><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)(fac * eps_p), (T)eps_u);
+            else
+                k_force<T, FORCE_KICK, false><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)(fac * eps_p), (T)eps_u);
             break;
         case FORCE_KICK_EXP:
-            k_force<T, FORCE_KICK_EXP><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)(fac * eps_p), (T)eps_u);
+            if (sh)
+                k_force<T, FORCE_KICK_EXP, true><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)(fac * eps_p),
+                                                                         (T)eps_u);
+            else
+                k_force<T, FORCE_KICK_EXP, false><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)(fac * eps_p),
+                                                                          (T)eps_u);
             break;
         default:
             return hipErrorInvalidValue;
