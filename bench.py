@@ -188,13 +188,14 @@ def stout_hmc_thermalized(gf, dims=DIMS, ntherm=6):
     }
 
 
-def production_stout_hmc(gf, dims=(48, 48, 48, 48), ntraj=50, check_every=10, seed=4848):
+def production_stout_hmc(gf, dims=(48, 48, 48, 48), ntraj=50, check_every=10, seed=4848, ntherm=0):
     """Production-size stout HMC: SU(3) 48^4, beta 6, 3 stout levels at rho 0.12, Omelyan 2MN
     16 steps, tau 1; FP32 links, momenta and forces (FP64 action/dH sums, Metropolis, CH
     coefficients); start U = exp(i omega.T), omega ~ N(0, 0.3^2); momenta refreshed on the device
-    (Philox stream (seed, trajectory)).  Every check_every-th trajectory is rerun in FP64 from the
-    same links and momenta (dH error); the last trajectory is run back with tau -> -tau from its
-    end point (reversibility |U(tau -> -tau) - U0|)."""
+    (Philox stream (seed, trajectory)).  ntherm thermalization trajectories are accepted
+    unconditionally first (momentum streams (seed, 10^6 + k)).  Every check_every-th trajectory
+    is rerun in FP64 from the same links and momenta (dH error); the last trajectory is run back
+    with tau -> -tau from its end point (reversibility |U(tau -> -tau) - U0|)."""
     gen = gf.generators
     t_all = time.perf_counter()
     L32 = gf.Lattice(dims, precision="f32")
@@ -202,6 +203,11 @@ def production_stout_hmc(gf, dims=(48, 48, 48, 48), ntraj=50, check_every=10, se
     L32.stout_config(3, 0.12, "auto")
     L64 = gf.Lattice(dims, precision="f64")
     L64.stout_config(3, 0.12, "auto")
+    therm = []
+    for k in range(ntherm):
+        L32.gaussian_momenta(seed, 10 ** 6 + k)
+        therm.append(L32.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True)[2])
+        log(f"[production] thermalization {k}: dH {therm[-1]:.4g}, plaquette {L32.plaquette():.6f}")
     rng = np.random.default_rng(seed + 1)
     times, dHs, acc, checks, plaq = [], [], 0, [], []
     for k in range(ntraj):
@@ -238,7 +244,8 @@ def production_stout_hmc(gf, dims=(48, 48, 48, 48), ntraj=50, check_every=10, se
     L32.close()
     L64.close()
     return {"lattice": "x".join(map(str, dims)), "sites": int(np.prod(dims)), "precision": "fp32 bulk",
-            "levels": 3, "rho": 0.12, "integrator": "2MN", "steps": 16, "tau": 1.0, "trajectories": ntraj,
+            "levels": 3, "rho": 0.12, "integrator": "2MN", "steps": 16, "tau": 1.0,
+            "thermalization_dH": therm, "trajectories": ntraj,
             "stout_memory_mode": mode, "s_per_traj_mean": float(np.mean(times)),
             "s_per_traj_min": float(np.min(times)), "acceptance": acc / ntraj, "dH": dHs,
             "plaquette": plaq, "fp64_reruns": checks,
@@ -257,6 +264,8 @@ def main():
     ap.add_argument("--stout-only", action="store_true", help="run only the stout-HMC and flow comparisons")
     ap.add_argument("--production", type=int, default=0, metavar="NTRAJ",
                     help="run only the 48^4 FP32-bulk stout HMC with NTRAJ trajectories")
+    ap.add_argument("--production-therm", type=int, default=0, metavar="N",
+                    help="unconditionally accepted thermalization trajectories before --production")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -265,7 +274,8 @@ def main():
 
     gf = __graft_entry__.load_package()
     if args.production:
-        print(json.dumps({"production_stout_hmc": production_stout_hmc(gf, ntraj=args.production)}))
+        print(json.dumps({"production_stout_hmc": production_stout_hmc(gf, ntraj=args.production,
+                                                                       ntherm=args.production_therm)}))
         return
     if args.stout_only:  # profiling aid: only the stout-HMC and flow comparisons
         print(json.dumps({"stout_hmc": stout_hmc_thermalized(gf, ntherm=args.stout_therm),
