@@ -141,6 +141,58 @@ def flow_compare(gf, dims=DIMS, steps=300, eps=0.01, every=10):
             "max_rel_t2E_diff_fp32": float((d / np.abs(out["f64"]["t2E"])).max())}
 
 
+def next_rows(gf, dims=DIMS, reps=5):
+    """Timing of the widened rows at 32^4 FP64 (FP32 for the heatbath as well): heatbath and
+    overrelaxation sweeps (8 checkerboard passes each), device Gaussian momenta and hot start
+    (Philox4x32-10), topological charge (clover, improved) and clover energy, and ILDG write +
+    read of the configuration through the host (64-bit payload, a temporary file).  Bytes are
+    algorithmic: a sweep reads and writes every link once (768 B/site FP64), momenta write
+    256 B/site, a hot start writes 384 B/site, a charge or energy reads 384 B/site."""
+    import tempfile
+
+    V = int(np.prod(dims))
+    out = {"lattice": "x".join(map(str, dims))}
+    for prec in ("f64", "f32"):
+        L = make_lattice(gf, dims, prec)
+        esz = 8 if prec == "f64" else 4
+        link_b = 48 * esz
+        ms_hb = time_kernel(L, lambda: L.heatbath(6.0, 11, 0), reps)
+        ms_or = time_kernel(L, lambda: L.overrelaxation(11, 1), reps)
+        row = {"heatbath_ms_per_sweep": ms_hb, "heatbath_link_updates_per_s": 4 * V / (ms_hb * 1e-3),
+               "heatbath_GBps": 2 * link_b * V / (ms_hb * 1e-3) / 1e9,
+               "overrelaxation_ms_per_sweep": ms_or,
+               "overrelaxation_GBps": 2 * link_b * V / (ms_or * 1e-3) / 1e9}
+        if prec == "f64":
+            ms_p = time_kernel(L, lambda: L.gaussian_momenta(12, 0), reps)
+            ms_h = time_kernel(L, lambda: L.hot_start(13), reps)
+            L.links_from_algebra(gf.generators.algebra_coefficients(dims, 0.3, 14))
+            ms_qc = time_kernel(L, lambda: L.topological_charge("clover"), reps)
+            ms_qi = time_kernel(L, lambda: L.topological_charge("improved"), reps)
+            ms_e = time_kernel(L, lambda: L.energy_density_clover(), reps)
+            row.update({
+                "gaussian_momenta_ms": ms_p, "gaussian_momenta_GBps": 32 * esz * V / (ms_p * 1e-3) / 1e9,
+                "hot_start_ms": ms_h, "hot_start_GBps": link_b * V / (ms_h * 1e-3) / 1e9,
+                "topological_charge_clover_ms_incl_readback": ms_qc,
+                "topological_charge_improved_ms_incl_readback": ms_qi,
+                "energy_density_clover_ms_incl_readback": ms_e,
+                "field_strength_GBps_clover": link_b * V / (ms_qc * 1e-3) / 1e9})
+            with tempfile.TemporaryDirectory() as d:
+                path = os.path.join(d, "conf.ildg")
+                L.sync()
+                t0 = time.perf_counter()
+                L.save_ildg(path, 64)
+                t1 = time.perf_counter()
+                L.load_ildg(path)
+                L.sync()
+                t2 = time.perf_counter()
+                nbytes = os.path.getsize(path)
+            row.update({"ildg_file_bytes": nbytes, "ildg_write_s": t1 - t0, "ildg_read_s": t2 - t1,
+                        "ildg_write_MBps": nbytes / (t1 - t0) / 1e6, "ildg_read_MBps": nbytes / (t2 - t1) / 1e6})
+        out[prec] = row
+        L.close()
+    return out
+
+
 def stout_hmc_thermalized(gf, dims=DIMS, ntherm=6):
     """3-level stout HMC (rho 0.12, 2MN 16 steps, tau 1): FP64 thermalization trajectories
     (accepted unconditionally), then one trajectory from the same links and momenta in FP64
@@ -262,6 +314,9 @@ def main():
     ap.add_argument("--no-stout", action="store_true", help="skip the stout-HMC s/traj lines")
     ap.add_argument("--stout-therm", type=int, default=6, help="FP64 thermalization trajectories")
     ap.add_argument("--stout-only", action="store_true", help="run only the stout-HMC and flow comparisons")
+    ap.add_argument("--no-next", action="store_true", help="skip the heatbath / random-field / field-strength / ILDG timings")
+    ap.add_argument("--next-only", action="store_true",
+                    help="run only the heatbath / random-field / field-strength / ILDG timings")
     ap.add_argument("--production", type=int, default=0, metavar="NTRAJ",
                     help="run only the 48^4 FP32-bulk stout HMC with NTRAJ trajectories")
     ap.add_argument("--production-therm", type=int, default=0, metavar="N",
@@ -276,6 +331,9 @@ def main():
     if args.production:
         print(json.dumps({"production_stout_hmc": production_stout_hmc(gf, ntraj=args.production,
                                                                        ntherm=args.production_therm)}))
+        return
+    if args.next_only:  # profiling aid: only the widened rows
+        print(json.dumps({"next_rows": next_rows(gf)}))
         return
     if args.stout_only:  # profiling aid: only the stout-HMC and flow comparisons
         print(json.dumps({"stout_hmc": stout_hmc_thermalized(gf, ntherm=args.stout_therm),
@@ -346,6 +404,10 @@ def main():
         log(f"[bench] stout {json.dumps(stout)}")
         flow = flow_compare(gf)
         log(f"[bench] flow {json.dumps(flow)}")
+    nxt = None
+    if not args.no_next:
+        nxt = next_rows(gf)
+        log(f"[bench] next rows {json.dumps(nxt)}")
 
     cpu = None
     if not args.no_cpu:
@@ -375,6 +437,7 @@ def main():
         "kernels": kernels,
         "stout_hmc": stout,
         "wilson_flow": flow,
+        "next_rows": nxt,
         "plaquette": plaq,
         "cpu_baseline": cpu,
     }
