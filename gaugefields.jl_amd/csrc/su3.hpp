@@ -323,6 +323,43 @@ __device__ __forceinline__ dc dca(dc a, dc b) { return {a.r + b.r, a.i + b.i}; }
 __device__ __forceinline__ dc dcsub(dc a, dc b) { return {a.r - b.r, a.i - b.i}; }
 __device__ __forceinline__ dc dcconj(dc a) { return {a.r, -a.i}; }
 
+// H^2 for a Hermitian H (upper triangle read; the result is Hermitian with a real diagonal)
+template <typename T>
+__device__ __forceinline__ M3<T> herm_sq(const M3<T>& H) {
+    auto h = [&](int i, int k) -> cx<T> {
+        if (i <= k) return H.m[3 * i + k];
+        cx<T> c = H.m[3 * k + i];
+        c.i = -c.i;
+        return c;
+    };
+    M3<T> S;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        T d = T(0);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const cx<T> a = h(i, k);
+            d = fma(a.r, a.r, d);
+            if (k != i) d = fma(a.i, a.i, d);
+        }
+        S.m[4 * i].r = d;
+        S.m[4 * i].i = T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = i + 1; j < 3; j++) {
+            cx<T> t;
+            t.r = t.i = T(0);
+#pragma unroll
+            for (int k = 0; k < 3; k++) cmac(t, h(i, k), h(k, j));
+            S.m[3 * i + j] = t;
+            S.m[3 * j + i].r = t.r;
+            S.m[3 * j + i].i = -t.i;
+        }
+    return S;
+}
+
 // c1 = tr(H^2)/2 and c0 = det H of a Hermitian traceless H, evaluated in double
 template <typename T>
 __device__ __forceinline__ void herm_invariants(const M3<T>& H, double& c0, double& c1) {
@@ -364,9 +401,9 @@ __device__ __forceinline__ void ch_coeffs(double c0, double c1, dc f[3], dc b1[3
         xi0 = sw / w;
         if (WANT_B) xi1 = cw / w2 - sw / (w2 * w);
     }
-    double su, cu, s2u, c2u;
+    double su, cu;
     sincos(u, &su, &cu);
-    sincos(2.0 * u, &s2u, &c2u);
+    const double s2u = 2.0 * su * cu, c2u = 1.0 - 2.0 * su * su;  // double-angle forms
     const dc emiu = {cu, -su}, e2iu = {c2u, s2u};
     // h0 = (u^2 - w^2) e2iu + emiu (8u^2 cos w + 2iu(3u^2 + w^2) xi0)
     dc h0 = dca(dcs(u2 - w2, e2iu), dcm(emiu, dc{8.0 * u2 * cw, 2.0 * u * (3.0 * u2 + w2) * xi0}));
@@ -444,7 +481,7 @@ template <typename T>
 __device__ __forceinline__ M3<T> expi_herm(const M3<T>& H) {
     double c0, c1;
     herm_invariants(H, c0, c1);
-    M3<T> H2 = mul(H, H);
+    M3<T> H2 = herm_sq(H);
     M3<T> E;
     if (c1 < GF_CH_EXP_TINY) {
 #pragma unroll
@@ -524,7 +561,7 @@ __device__ __forceinline__ M3<T> cdexpqdq(const M3<T>& C, const M3<T>& Q) {
     }
     dc f[3], b1[3], b2[3];
     ch_coeffs<true>(c0, c1, f, b1, b2);
-    M3<T> H2 = mul(H, H);
+    M3<T> H2 = herm_sq(H);
     // traces tr(C), tr(C H), tr(C H^2) in double
     dc t0 = {0.0, 0.0}, t1 = {0.0, 0.0}, t2 = {0.0, 0.0};
 #pragma unroll
@@ -574,7 +611,7 @@ __device__ __forceinline__ void exp_dexp(const M3<T>& Q, const M3<T>& X, M3<T>& 
     }
     double c0, c1;
     herm_invariants(H, c0, c1);
-    const M3<T> H2 = mul(H, H);
+    const M3<T> H2 = herm_sq(H);
     if (c1 < GF_CH_DER_SERIES) {
         if (c1 < GF_CH_EXP_TINY) {
 #pragma unroll
