@@ -66,13 +66,23 @@ __global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict_
 }
 
 // K2: dSdU_mu = coeff * Sigma_mu
-template <typename T>
+// SHARED: upper-staple products shared between directions through LDS (V % 64 == 0)
+template <typename T, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U, T coeff, T* __restrict__ D) {
+    __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
     const SiteMu sm = site_mu_index(g);
-    if (sm.s >= g.V) return;
+    int s;
+    if (!stencil_site<SHARED>(g, sm, s)) return;
     int x[4];
-    decode(g, sm.s, x);
-    stm(D, (unsigned)g.V, sm.mu, (unsigned)sm.s, scal(coeff, staple_sigma(g, U, (unsigned)g.V, sm.s, x, sm.mu)));
+    decode(g, s, x);
+    M3<T> sig;
+    if constexpr (SHARED) {
+        R6<T> own;
+        sig = staple_sigma_shared(g, U, (unsigned)g.V, s, x, sm.mu, own, xs);
+    } else {
+        sig = staple_sigma(g, U, (unsigned)g.V, s, x, sm.mu);
+    }
+    stm<T, GF_ST_NT>(D, (unsigned)g.V, sm.mu, (unsigned)s, scal(coeff, sig));
 }
 
 // K3 (+K4): F_mu = -(coeff/3) TA(U_mu Sigma_mu)   (md_force!, molecular_dynamics.jl:251-267)
@@ -262,7 +272,10 @@ hipError_t launch_momenta_dot(const Geo& g, const T* P, double* partial, double*
 }
 template <typename T>
 hipError_t launch_dsdu(const Geo& g, const T* U, double coeff, T* D, hipStream_t st) {
-    k_dsdu<T><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, (T)coeff, D);
+    if (g.V % 64 == 0)
+        k_dsdu<T, true><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, (T)coeff, D);
+    else
+        k_dsdu<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, (T)coeff, D);
     return hipGetLastError();
 }
 template <typename T>
