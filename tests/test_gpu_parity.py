@@ -321,3 +321,33 @@ def test_stout_auto_mode_stores_when_it_fits(Lat, oracle):
     L.force(COEFF)
     F = oracle.stout_force(D4, U, 2, 0.1, COEFF)
     assert maxdiff(L.download_force(), F) < 1e-12 * np.abs(F).max()
+
+
+# ------------------------------------------------------------------ volumes that are not a multiple of 64
+# (the stencil kernels then run their unshared form: no workgroup barrier, no LDS products)
+
+
+@pytest.mark.parametrize("mode", ["store", "recompute"])
+def test_unaligned_volume_stout_flow_md(Lat, oracle, mode):
+    dims = (3, 4, 5, 2)  # V = 120
+    print(f"[parity] unaligned volume {dims} stout {mode}", flush=True)
+    L = Lat(dims)
+    U = alg_links(oracle, dims, 0.3, 71)
+    P = gen.gaussian_momenta(dims, 72)
+    L.upload_links(U)
+    L.stout_config(2, 0.1, mode)
+    assert maxdiff(L.stout_smear(), oracle.stout_smear(dims, U, 2, 0.1)) < 1e-14
+    L.force(COEFF)
+    F = oracle.stout_force(dims, U, 2, 0.1, COEFF)
+    assert maxdiff(L.download_force(), F) < 1e-12 * np.abs(F).max()
+    L.stout_config(0, 0.0, "store")
+    L.upload_momenta(P)
+    H = L.md_trajectory(COEFF, 3, 0.3, "2MN", fused=True)
+    U2, P2 = U.copy(), P.copy()
+    Href = oracle.md_trajectory(dims, U2, P2, COEFF, 3, 0.3, "2MN")
+    assert abs(H[2] - Href[2]) < 1e-12 * abs(Href[0])
+    assert maxdiff(L.download_links(), U2) < 1e-12
+    E = L.flow(0.02, 4, 2)
+    Eref = oracle.flow(dims, U2, 0.02, 4, 2)
+    assert np.all(np.abs(E - Eref) < 1e-12 * np.abs(Eref))
+    assert maxdiff(L.download_links(), U2) < 1e-13
