@@ -121,20 +121,23 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
 // Z = U_a(y-b)^dag U_b(y-b),
 //   upper = X Lam_a(y+b) U_b(y)^dag + S Lam_b(y) - Lam_b(y+a) S
 //   lower = U_b(y+a-b)^dag [U_a(y-b)^dag (Lam_a(y-b) - Lam_b(y-b)) U_b(y-b) + Lam_b(y+a-b) Z]
-// Six links and six 8-real Lam per plane instead of ten links and six 18-real G.
+// Six links and six 8-real Lam per plane instead of ten links and six 18-real G.  Lam = i H with
+// H Hermitian (its 8 coefficients): products with Lam are i times Hermitian-structured products.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ M3<T> lam_of(const T* __restrict__ Lc, unsigned V, int mu, unsigned s) {
+__device__ __forceinline__ Herm<T> herm_at(const T* __restrict__ Lc, unsigned V, int mu, unsigned s) {
     T c[8];
     ldp(Lc, V, mu, s, c);
-    const M3<T> H = herm_from_coeffs(c, T(1));  // sum_a c_a lambda_a / 2
-    M3<T> A;
+    return herm_of_coeffs(c);
+}
+// c += i m
+template <typename T>
+__device__ __forceinline__ void acc_i(M3<T>& c, const M3<T>& m) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) {  // i H
-        A.m[k].r = -H.m[k].i;
-        A.m[k].i = H.m[k].r;
+    for (int k = 0; k < 9; k++) {
+        c.m[k].r -= m.m[k].i;
+        c.m[k].i += m.m[k].r;
     }
-    return A;
 }
 
 template <typename T>
@@ -156,35 +159,39 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* _
         const int ymb0 = y + dmb, ypamb0 = ypa + dmb;
         // each group loads its links through its own opaque index copies, so the compiler
         // issues them group by group instead of hoisting all of them
-        {  // upper
+        {  // upper: i [X H_a(y+b)] U_b(y)^dag + i [S H_b(y) - H_b(y+a) S]
             int y0 = y, y1 = ypa, ypb = ypb0;
             asm volatile("" : "+v"(y0), "+v"(y1), "+v"(ypb));
-            const M3<T> X = mul_nd(ldu(U, V, b, y1), ldu(U, V, a, ypb));
+            const M3<T> X = su3_mul_nd(ldr(U, V, b, y1), ldu(U, V, a, ypb));
             const M3<T> ub = ldu(U, V, b, y0);
-            acc_mul_nd(acc, mul(X, lam_of(Lc, V, a, ypb)), ub);
+            acc_mul_nd(acc, times_i(mul_mh(X, herm_at(Lc, V, a, ypb))), ub);
             const M3<T> S = mul_nd(X, ub);
-            acc_mul(acc, S, lam_of(Lc, V, b, y0));
-            const M3<T> t = mul(lam_of(Lc, V, b, y1), S);
+            M3<T> t = mul_mh(S, herm_at(Lc, V, b, y0));
+            const M3<T> t2 = mul_hm(herm_at(Lc, V, b, y1), S);
 #pragma unroll
             for (int k = 0; k < 9; k++) {
-                acc.m[k].r -= t.m[k].r;
-                acc.m[k].i -= t.m[k].i;
+                t.m[k].r -= t2.m[k].r;
+                t.m[k].i -= t2.m[k].i;
             }
+            acc_i(acc, t);
         }
-        {  // lower
+        {  // lower: U_b(y+a-b)^dag i [U_a(y-b)^dag H_d U_b(y-b) + H_b(y+a-b) Z], H_d = H_a(y-b) - H_b(y-b)
             int ymb = ymb0, ypamb = ypamb0;
             asm volatile("" : "+v"(ymb), "+v"(ypamb));
             const M3<T> ua = ldu(U, V, a, ymb), ub = ldu(U, V, b, ymb);
-            M3<T> d = lam_of(Lc, V, a, ymb);
-            const M3<T> lb = lam_of(Lc, V, b, ymb);
+            T ca[8], cb[8];
+            ldp(Lc, V, a, ymb, ca);
+            ldp(Lc, V, b, ymb, cb);
+#pragma unroll
+            for (int k = 0; k < 8; k++) ca[k] -= cb[k];
+            M3<T> t = mul(mul_dh(ua, herm_of_coeffs(ca)), ub);
+            const M3<T> t2 = mul_hm(herm_at(Lc, V, b, ypamb), mul_dn(ua, ub));
 #pragma unroll
             for (int k = 0; k < 9; k++) {
-                d.m[k].r -= lb.m[k].r;
-                d.m[k].i -= lb.m[k].i;
+                t.m[k].r += t2.m[k].r;
+                t.m[k].i += t2.m[k].i;
             }
-            M3<T> t = mul(mul_dn(ua, d), ub);
-            acc_mul(t, lam_of(Lc, V, b, ypamb), mul_dn(ua, ub));
-            acc_mul_dn(acc, ldu(U, V, b, ypamb), t);
+            acc_mul_dn(acc, ldu(U, V, b, ypamb), times_i(t));
         }
     }
     M3<T> dn = ldm(Din, V, a, y);
