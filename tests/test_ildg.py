@@ -148,6 +148,9 @@ def test_errors(ildg, gf, tmp_path):
         ildg.read_ildg(tmp_path / "junk.ildg", (2, 2, 2, 2))
     with pytest.raises(ValueError):
         ildg.write_ildg(tmp_path / "bad.ildg", (2, 2, 2, 2), U[:, :8], 64)
+    # an unwritable destination (the reference broadcasts such root-rank I/O failures, API.jl:545-561)
+    with pytest.raises(GFError, match="cannot"):
+        ildg.write_ildg(tmp_path / "no_such_dir" / "c.ildg", (2, 2, 2, 2), U, 64)
 
 
 def test_api_header(gf):
