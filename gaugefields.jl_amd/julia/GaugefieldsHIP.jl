@@ -12,6 +12,8 @@ kernels.  The methods specialize the reference's own entry points:
   update_gaugefields! / update_momenta! / md_trajectory!  (molecular_dynamics.jl:513-730)
   flow!(U, ::Gradientflow)   (gradientflow.jl:171-238)
   back_prop / STOUT_Layer    (Abstractsmearing.jl:352-411, stout_fast.jl:223-274)
+  gauss_distribution!, hot start, heatbath! / overrelaxation!, topological_charge,
+  calculate_energy_density, ILDG load_gaugefield! / save_binarydata
 
 This file is not executed in this repository (there is no Julia toolchain here); the same
 C ABI is exercised by the Python ctypes mirror in ../api.py and the tests.
@@ -120,6 +122,50 @@ end
 stout_config!(U::HIPConfiguration, nlev::Integer, rho::Real; recompute::Bool=false) =
     _check(U, ccall((:gf_stout_config, libgfhip), Cint, (Ptr{Cvoid}, Cint, Cdouble, Cint),
                     U.ctx, Cint(nlev), rho, Cint(recompute ? 1 : 0)))
+
+# Seeded fields drawn on the device (Philox4x32-10; stream key = seed, sweep in the counter):
+# gauss_distribution! / randomGaugefields (TA_gaugefields_4D_MPILattice.jl:157-193,
+# gaugefields_4D_MPILattice.jl:430-472)
+gaussian_momenta!(U::HIPConfiguration; sigma::Real=1.0, seed::Integer=0, sweep::Integer=0) =
+    _check(U, ccall((:gf_gaussian_momenta, libgfhip), Cint, (Ptr{Cvoid}, Cdouble, Culonglong, Culonglong),
+                    U.ctx, sigma, seed, sweep))
+
+hot_start!(U::HIPConfiguration; seed::Integer=0) =
+    _check(U, ccall((:gf_hot_start, libgfhip), Cint, (Ptr{Cvoid}, Culonglong), U.ctx, seed))
+
+# heatbath! / overrelaxation! for the plaquette action (heatbath/heatbathmodule.jl:50-200,
+# 1248-1321, 2119-2276); ITERATION_MAX above 65534 acts as 65534
+function heatbath!(U::HIPConfiguration, beta::Real; seed::Integer=0, sweep::Integer=0,
+                   ITERATION_MAX::Integer=65534)
+    failures = Ref{Clonglong}(0)
+    _check(U, ccall((:gf_heatbath, libgfhip), Cint,
+                    (Ptr{Cvoid}, Cdouble, Culonglong, Culonglong, Cint, Ref{Clonglong}),
+                    U.ctx, beta, seed, sweep, Cint(min(ITERATION_MAX, 65534)), failures))
+    return Int(failures[])
+end
+
+overrelaxation!(U::HIPConfiguration; seed::Integer=0, sweep::Integer=0) =
+    _check(U, ccall((:gf_overrelaxation, libgfhip), Cint, (Ptr{Cvoid}, Culonglong, Culonglong),
+                    U.ctx, seed, sweep))
+
+# topological_charge / calculate_energy_density (AbstractGaugefields.jl:1172-1471;
+# samples/measurements/energydensity.jl:32-76)
+const GF_TOPO_METHODS = Dict(:plaquette => Cint(0), :clover => Cint(1), :improved => Cint(2))
+
+function topological_charge(U::HIPConfiguration; method::Symbol=:plaquette)
+    haskey(GF_TOPO_METHODS, method) ||
+        throw(ArgumentError("supported topological_charge methods are :plaquette, :clover and :improved"))
+    Q = Ref{Float64}(0)
+    _check(U, ccall((:gf_topological_charge, libgfhip), Cint, (Ptr{Cvoid}, Cint, Ref{Float64}),
+                    U.ctx, GF_TOPO_METHODS[method], Q))
+    return Q[]
+end
+
+function calculate_energy_density(U::HIPConfiguration)
+    E = Ref{Float64}(0)
+    _check(U, ccall((:gf_energy_density_clover, libgfhip), Cint, (Ptr{Cvoid}, Ref{Float64}), U.ctx, E))
+    return E[]
+end
 
 # ILDG files (ildg_format.jl:686-969): host I/O in the library, no CLIME executables
 function _check_ildg(rc::Cint)
