@@ -154,7 +154,7 @@ const GF_TOPO_METHODS = Dict(:plaquette => Cint(0), :clover => Cint(1), :improve
 
 function topological_charge(U::HIPConfiguration; method::Symbol=:plaquette)
     haskey(GF_TOPO_METHODS, method) ||
-        throw(ArgumentError("supported topological_charge methods are :plaquette, :clover and :improved"))
+        throw(ArgumentError("supported topological_charge methods are :plaquette, :clover, and :improved"))
     Q = Ref{Float64}(0)
     _check(U, ccall((:gf_topological_charge, libgfhip), Cint, (Ptr{Cvoid}, Cint, Ref{Float64}),
                     U.ctx, GF_TOPO_METHODS[method], Q))
