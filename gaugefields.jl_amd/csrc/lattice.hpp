@@ -27,8 +27,10 @@ struct Geo {
     // selects plain linear group order (extents that are not powers of two)
     int mbits[3];
     // t-sweep tiles (used when tile[0] > 0): the (xy-group, z) plane is cut into 8 tiles of
-    // tile[0] x tile[1] columns, one per XCD, and each XCD walks its columns slice by slice in t
+    // tile[0] x tile[1] columns, one per XCD; each XCD splits its tile along z into `sub`
+    // sub-tiles and walks them one after another, each slice by slice in t
     int tile[2];
+    int sub;
     // division by n[0], n[1], n[2] (set by set_divisors): q = mulhi(r, dm) >> ds, or r >> ds
     // when dm == 0 (a power of two)
     unsigned dm[3];
@@ -90,12 +92,16 @@ __device__ __forceinline__ int site_index() { return (int)(xcd_block() * blockDi
 // (xy-group, z, t) (power-of-two extents) or linear order.
 __host__ __device__ __forceinline__ int tsweep_group(const Geo& g, unsigned b, unsigned nb) {
     const int ngy = g.n[0] * g.n[1] / 64;
-    const unsigned per = nb / 8u, xcd = b / per, loc = b % per;
+    const unsigned per = nb / 8u, xcd = b / per, loc0 = b % per;
     const int nty = ngy / g.tile[0];
-    const int cols = g.tile[0] * g.tile[1];
+    const int subz = g.tile[1] / g.sub;  // z-planes of a sub-tile
+    const int cols = g.tile[0] * subz;
+    const unsigned persub = per / (unsigned)g.sub;
+    const int si = (int)(loc0 / persub);
+    const unsigned loc = loc0 % persub;
     const int t = (int)loc / cols, c = (int)loc % cols;
     const int gy = (int)(xcd % (unsigned)nty) * g.tile[0] + c % g.tile[0];
-    const int z = (int)(xcd / (unsigned)nty) * g.tile[1] + c / g.tile[0];
+    const int z = (int)(xcd / (unsigned)nty) * g.tile[1] + si * subz + c / g.tile[0];
     return gy + ngy * (z + g.n[2] * t);
 }
 

@@ -280,11 +280,13 @@ def test_fp32_bulk_against_fp64_oracle(Lat, oracle):
 # ------------------------------------------------------------------ workgroup orderings
 # Extents whose xy-plane holds a multiple of 64 sites run the stencil kernels in a non-linear
 # group order: the t-sweep tiles when the (xy-group, z) plane splits into 8 tiles ((8,8,8,8),
-# (16,8,4,8), (24,8,8,2)), else 4D Morton order for power-of-two extents ((32,4,2,4),
-# (8,8,2,2)); these cases pin both paths.
+# (16,8,4,8), (24,8,8,2); (32,32,16,2) takes 2 x 16 tiles split into 2 sub-tiles of 2 x 8
+# columns), else 4D Morton order for power-of-two extents ((32,4,2,4), (8,8,2,2)); these
+# cases pin both paths.
 
 
-@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (16, 8, 4, 8), (24, 8, 8, 2), (32, 4, 2, 4), (8, 8, 2, 2)])
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (16, 8, 4, 8), (24, 8, 8, 2), (32, 32, 16, 2), (32, 4, 2, 4),
+                                  (8, 8, 2, 2)])
 def test_morton_path_md_and_flow(Lat, oracle, dims):
     L = Lat(dims)
     U = alg_links(oracle, dims, 0.3, 61)
