@@ -42,6 +42,8 @@ struct gf_ctx {
     void* D = nullptr;
     void* D2 = nullptr;
     void* G = nullptr;
+    // FP32 bulk: FP64 copies of the links (and smeared levels, ping-pong) for the Hamiltonian
+    void* W64[2] = {nullptr, nullptr};
     // scratch for layout conversion and reductions
     double* staging = nullptr;  // 72 V doubles
     double* partial = nullptr;
@@ -263,6 +265,27 @@ int kick_exp(gf_ctx* c, double coeff, double eps_p, double eps_u) {
     return GF_OK;
 }
 
+// Sum Re tr P of the (smeared) links in FP64 arithmetic.  In FP32 bulk the stored links are
+// widened to FP64 and smeared in FP64 (no C kept): the FP32 smearing chain's own rounding would
+// add configuration-dependent noise of ~2e-3 to the smeared action at 32^4, more than the
+// deviation the FP32 trajectory itself causes; the forces stay FP32.
+int action_plaq_fp64(gf_ctx* c, double* plaq) {
+    if (c->prec == GF_F64) {
+        if (c->nlev > 0) TRY(stout_forward_all(c));
+        return plaq_of(c, (void*)top_links(c), plaq);
+    }
+    const size_t b64 = (size_t)48 * c->V * sizeof(double);
+    TRY(alloc(c, &c->W64[0], b64));
+    if (c->nlev > 0) TRY(alloc(c, &c->W64[1], b64));
+    HIPCHK(c, launch_links_widen((const float*)c->U, (double*)c->W64[0], c->V, c->stream));
+    int cur = 0;
+    for (int l = 0; l < c->nlev; l++, cur ^= 1)
+        HIPCHK(c, launch_stout_forward<double>(c->geo, (const double*)c->W64[cur], (double*)c->W64[cur ^ 1], nullptr,
+                                               c->rho, c->stream));
+    HIPCHK(c, launch_plaquette<double>(c->geo, (const double*)c->W64[cur], c->partial, c->dres, c->stream));
+    return read_scalar(c, plaq);
+}
+
 int hamiltonian(gf_ctx* c, double coeff, double* H) {
     double pp = 0.0, plaq = 0.0;
     TRY(dispatch(c, [&](auto* tag) -> int {
@@ -271,8 +294,7 @@ int hamiltonian(gf_ctx* c, double coeff, double* H) {
         return GF_OK;
     }));
     TRY(read_scalar(c, &pp));
-    if (c->nlev > 0) TRY(stout_forward_all(c));
-    TRY(plaq_of(c, (void*)top_links(c), &plaq));
+    TRY(action_plaq_fp64(c, &plaq));
     *H = -2.0 * coeff * plaq / 3.0 + 0.5 * pp;
     return GF_OK;
 }
@@ -398,7 +420,7 @@ int gf_destroy(gf_ctx* c) {
     if (!c) return GF_EINVAL;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void** bufs[] = {&c->U, &c->Ualt, &c->Ubak, &c->P, &c->F, &c->A, &c->D, &c->D2, &c->G};
+    void** bufs[] = {&c->U, &c->Ualt, &c->Ubak, &c->P, &c->F, &c->A, &c->D, &c->D2, &c->G, &c->W64[0], &c->W64[1]};
     for (void** b : bufs)
         if (*b) (void)hipFree(*b);
     for (int l = 0; l < GF_MAX_LEV; l++) {

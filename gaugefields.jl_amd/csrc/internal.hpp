@@ -44,6 +44,7 @@ hipError_t launch_force_from_deriv(int mode, const Geo& g, const T* U, const T* 
                                    hipStream_t st);
 // reference-layout (column-major 3x3 per link) <-> device field; general = false: SU(3) link
 // field (rows 0 and 1 kept, row 2 rebuilt on the way out), true: general matrix field
+hipError_t launch_links_widen(const float* Uf, double* Ud, int V, hipStream_t st);
 template <typename T>
 hipError_t launch_links_from_ref(const double* ref, T* soa, int V, bool general, hipStream_t st);
 template <typename T>

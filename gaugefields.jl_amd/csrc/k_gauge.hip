@@ -166,6 +166,25 @@ __global__ __launch_bounds__(kBlock) void k_force_from_deriv(Geo g, const T* __r
     }
 }
 
+// FP32-bulk link field -> FP64 link field (the two stored rows, exactly; row 2 is rebuilt on
+// every load in either precision)
+__global__ __launch_bounds__(kBlock) void k_links_widen(const float* __restrict__ Uf, double* __restrict__ Ud,
+                                                        int V) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= V) return;
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) {
+        const R6<float> r = ldr(Uf, (unsigned)V, mu, (unsigned)s);
+        M3<double> m;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            m.m[k].r = (double)r.v[k].x;
+            m.m[k].i = (double)r.v[k].y;
+        }
+        stu(Ud, (unsigned)V, mu, (unsigned)s, m);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // layout conversion between the reference's (3,3,X,Y,Z,T) complex column-major arrays
 // (FP64, per direction) and the SoA planes
@@ -324,6 +343,10 @@ hipError_t launch_force_from_deriv(int mode, const Geo& g, const T* U, const T* 
         k_force_from_deriv<T, FORCE_KICK><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, D, FP, (T)eps);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+hipError_t launch_links_widen(const float* Uf, double* Ud, int V, hipStream_t st) {
+    k_links_widen<<<grid_for(V), kBlock, 0, st>>>(Uf, Ud, V);
     return hipGetLastError();
 }
 template <typename T>
