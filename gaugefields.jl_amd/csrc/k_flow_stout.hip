@@ -13,7 +13,7 @@ namespace gf {
 // SHARED: upper-staple products shared between directions through LDS (V % 64 == 0)
 template <typename T, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restrict__ W, T* __restrict__ Wout,
-                                                       T* __restrict__ A, T alpha, T beta) {
+                                                       T* __restrict__ A, double alpha, double beta) {
     __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
     const SiteMu sm = site_mu_index(g);
     const int mu = sm.mu;
@@ -31,13 +31,13 @@ __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restric
         T a[8];
         ldp(A, V, mu, s, a);
 #pragma unroll
-        for (int k = 0; k < 8; k++) c[k] = fma(alpha, c[k], beta * a[k]);
+        for (int k = 0; k < 8; k++) c[k] = (T)fma(alpha, (double)c[k], beta * (double)a[k]);
     } else {
 #pragma unroll
-        for (int k = 0; k < 8; k++) c[k] = alpha * c[k];
+        for (int k = 0; k < 8; k++) c[k] = scale1(alpha, c[k]);
     }
     stp<T, GF_ST_NT>(A, V, mu, s, c);
-    const M3<T> wn = exp_coeffs_times(c, T(1), w);
+    const M3<T> wn = exp_coeffs_times(c, 1.0, w);
     stu<T, GF_ST_NT>(Wout, V, mu, s, wn);
 }
 
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restric
 // ---------------------------------------------------------------------------
 template <typename T, bool STORE, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_stout_forward(Geo g, const T* __restrict__ Uin, T* __restrict__ Uout,
-                                                          T* __restrict__ Cst, T rho) {
+                                                          T* __restrict__ Cst, double rho) {
     __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
     const SiteMu sm = site_mu_index(g);
     const int mu = sm.mu;
@@ -58,10 +58,18 @@ __global__ __launch_bounds__(kBlock) void k_stout_forward(Geo g, const T* __rest
     decode(g, s, x);
     const unsigned V = g.V;
     R6<T> own;
-    const M3<T> C = scal(rho, dag(staple_sum_own<T, SHARED>(g, Uin, V, s, x, mu, own, xs)));
+    const M3<T> sig = staple_sum_own<T, SHARED>(g, Uin, V, s, x, mu, own, xs);
     const M3<T> u = full_of(own);
-    stu<T, GF_ST_NT>(Uout, V, mu, s, exp_antiherm_times(ta_mat(mul_nd(C, u)), u));
-    if (STORE) stm<T, GF_ST_NT>(Cst, V, mu, s, C);
+    if constexpr (std::is_same<T, float>::value) {
+        // C = rho Sigma^dag rounded once (the backward pass reads this C), Q and exp(Q) U in double
+        const M3<T> C = round_link<T>(scal(rho, dag(to_f64(sig))));
+        stu<T, GF_ST_NT>(Uout, V, mu, s, exp_antiherm_times(ta_mat(mul_nd(to_f64(C), to_f64(u))), u));
+        if (STORE) stm<T, GF_ST_NT>(Cst, V, mu, s, C);
+    } else {
+        const M3<T> C = scal(rho, dag(sig));
+        stu<T, GF_ST_NT>(Uout, V, mu, s, exp_antiherm_times(ta_mat(mul_nd(C, u)), u));
+        if (STORE) stm<T, GF_ST_NT>(Cst, V, mu, s, C);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -74,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_stout_forward(Geo g, const T* __rest
 template <typename T, bool STORED, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __restrict__ Uin,
                                                              const T* __restrict__ Cst, const T* __restrict__ Dout,
-                                                             T* __restrict__ Din, T* __restrict__ Lc, T rho) {
+                                                             T* __restrict__ Din, T* __restrict__ Lc, double rho) {
     __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
     const SiteMu sm = site_mu_index(g);
     const int mu = sm.mu;
@@ -89,7 +97,7 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
     } else {  // the same staple evaluation (and rounding) as the forward kernel
         int x[4];
         decode(g, s, x);
-        C = scal(rho, dag(staple_sum_own<T, SHARED>(g, Uin, V, s, x, mu, own, xs)));
+        C = round_link<T>(scal(rho, dag(to_f64(staple_sum_own<T, SHARED>(g, Uin, V, s, x, mu, own, xs)))));
     }
     M3<T> E, L;
     {
@@ -142,7 +150,7 @@ __device__ __forceinline__ void acc_i(M3<T>& c, const M3<T>& m) {
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* __restrict__ U,
-                                                               const T* __restrict__ Lc, T* __restrict__ Din, T rho) {
+                                                               const T* __restrict__ Lc, T* __restrict__ Din, double rho) {
     const SiteMu sm = site_mu_index(g);
     const int y = sm.s, a = sm.mu;
     if (y >= g.V) return;
@@ -197,8 +205,8 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* _
     M3<T> dn = ldm(Din, V, a, y);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        dn.m[k].r = fma(rho, acc.m[k].r, dn.m[k].r);
-        dn.m[k].i = fma(rho, acc.m[k].i, dn.m[k].i);
+        dn.m[k].r = fma1(rho, acc.m[k].r, dn.m[k].r);
+        dn.m[k].i = fma1(rho, acc.m[k].i, dn.m[k].i);
     }
     stm<T, GF_ST_NT>(Din, V, a, y, dn);
 }
@@ -207,9 +215,9 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* _
 template <typename T>
 hipError_t launch_flow_stage(const Geo& g, const T* W, T* Wout, T* A, double alpha, double beta, hipStream_t st) {
     if (g.V % 64 == 0)
-        k_flow_stage<T, true><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, W, Wout, A, (T)alpha, (T)beta);
+        k_flow_stage<T, true><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, W, Wout, A, alpha, beta);
     else
-        k_flow_stage<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, W, Wout, A, (T)alpha, (T)beta);
+        k_flow_stage<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, W, Wout, A, alpha, beta);
     return hipGetLastError();
 }
 template <typename T>
@@ -217,13 +225,13 @@ hipError_t launch_stout_forward(const Geo& g, const T* Uin, T* Uout, T* Cst, dou
     const int nb = grid_sites_mu(g.V);
     const bool sh = g.V % 64 == 0;
     if (Cst && sh)
-        k_stout_forward<T, true, true><<<nb, kBlock, 0, st>>>(g, Uin, Uout, Cst, (T)rho);
+        k_stout_forward<T, true, true><<<nb, kBlock, 0, st>>>(g, Uin, Uout, Cst, rho);
     else if (Cst)
-        k_stout_forward<T, true, false><<<nb, kBlock, 0, st>>>(g, Uin, Uout, Cst, (T)rho);
+        k_stout_forward<T, true, false><<<nb, kBlock, 0, st>>>(g, Uin, Uout, Cst, rho);
     else if (sh)
-        k_stout_forward<T, false, true><<<nb, kBlock, 0, st>>>(g, Uin, Uout, nullptr, (T)rho);
+        k_stout_forward<T, false, true><<<nb, kBlock, 0, st>>>(g, Uin, Uout, nullptr, rho);
     else
-        k_stout_forward<T, false, false><<<nb, kBlock, 0, st>>>(g, Uin, Uout, nullptr, (T)rho);
+        k_stout_forward<T, false, false><<<nb, kBlock, 0, st>>>(g, Uin, Uout, nullptr, rho);
     return hipGetLastError();
 }
 template <typename T>
@@ -231,14 +239,14 @@ hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const
                                  hipStream_t st) {
     const int nb = grid_sites_mu(g.V);
     if (Cst)
-        k_stout_back_local<T, true, false><<<nb, kBlock, 0, st>>>(g, Uin, Cst, Dout, Din, Lc, (T)rho);
+        k_stout_back_local<T, true, false><<<nb, kBlock, 0, st>>>(g, Uin, Cst, Dout, Din, Lc, rho);
     else if (g.V % 64 == 0)  // recompute C exactly as k_stout_forward<T, *, true> formed it
-        k_stout_back_local<T, false, true><<<nb, kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, Lc, (T)rho);
+        k_stout_back_local<T, false, true><<<nb, kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, Lc, rho);
     else
-        k_stout_back_local<T, false, false><<<nb, kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, Lc, (T)rho);
+        k_stout_back_local<T, false, false><<<nb, kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, Lc, rho);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_stout_back_stencil<T><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Lc, Din, (T)rho);
+    k_stout_back_stencil<T><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Lc, Din, rho);
     return hipGetLastError();
 }
 

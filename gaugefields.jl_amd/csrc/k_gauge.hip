@@ -68,7 +68,7 @@ __global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict_
 // K2: dSdU_mu = coeff * Sigma_mu
 // SHARED: upper-staple products shared between directions through LDS (V % 64 == 0)
 template <typename T, bool SHARED>
-__global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U, T coeff, T* __restrict__ D) {
+__global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U, double coeff, T* __restrict__ D) {
     __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
     const SiteMu sm = site_mu_index(g);
     int s;
@@ -82,7 +82,12 @@ __global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U,
     } else {
         sig = staple_sigma(g, U, (unsigned)g.V, s, x, sm.mu);
     }
-    stm<T, GF_ST_NT>(D, (unsigned)g.V, sm.mu, (unsigned)s, scal(coeff, sig));
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        sig.m[k].r = scale1(coeff, sig.m[k].r);
+        sig.m[k].i = scale1(coeff, sig.m[k].i);
+    }
+    stm<T, GF_ST_NT>(D, (unsigned)g.V, sm.mu, (unsigned)s, sig);
 }
 
 // K3 (+K4): F_mu = -(coeff/3) TA(U_mu Sigma_mu)   (md_force!, molecular_dynamics.jl:251-267)
@@ -92,7 +97,7 @@ __global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U,
 // SHARED: upper-staple products shared between directions through LDS (V % 64 == 0)
 template <typename T, int MODE, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U, T* __restrict__ FP,
-                                                  T* __restrict__ Unew, T fac, T eps_u) {
+                                                  T* __restrict__ Unew, double fac, double eps_u) {
     __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
     const SiteMu sm = site_mu_index(g);
     const int mu = sm.mu;
@@ -108,13 +113,13 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U
     ta_coeffs(mul(umu, sig), c);
     if (MODE == FORCE_STORE) {
 #pragma unroll
-        for (int a = 0; a < 8; a++) c[a] *= fac;
+        for (int a = 0; a < 8; a++) c[a] = scale1(fac, c[a]);
         stp<T, GF_ST_NT>(FP, V, mu, s, c);
     } else {
         T p[8];
         ldp(FP, V, mu, s, p);
 #pragma unroll
-        for (int a = 0; a < 8; a++) p[a] = fma(fac, c[a], p[a]);
+        for (int a = 0; a < 8; a++) p[a] = fma1(fac, c[a], p[a]);
         stp<T, GF_ST_NT>(FP, V, mu, s, p);
         if (MODE == FORCE_KICK_EXP) {
             const M3<T> un = exp_coeffs_times(p, eps_u, umu);
@@ -126,7 +131,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U
 // K4: U_mu <- exp(eps P_mu) U_mu  (exptU! + mul!, molecular_dynamics.jl:513-531)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_exp_update(Geo g, const T* __restrict__ Uin, T* __restrict__ Uout,
-                                                       const T* __restrict__ P, T eps) {
+                                                       const T* __restrict__ P, double eps) {
     // purely streaming: one thread per site walking its four links keeps more loads in flight
     const int s = site_index();
     if (s >= g.V) return;
@@ -144,24 +149,24 @@ __global__ __launch_bounds__(kBlock) void k_exp_update(Geo g, const T* __restric
 // test/HMCstout_test_nowing.jl:99-118); store or kick
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_force_from_deriv(Geo g, const T* __restrict__ U, const T* __restrict__ D,
-                                                             T* __restrict__ FP, T eps) {
+                                                             T* __restrict__ FP, double eps) {
     const SiteMu sm = site_mu_index(g);
     const int s = sm.s, mu = sm.mu;
     if (s >= g.V) return;
     const unsigned V = g.V;
-    const T fac = T(-1.0 / 3.0);
+    constexpr double fac = -1.0 / 3.0;
     T c[8];
     ta_coeffs(mul(ldu(U, V, mu, s), ldm(D, V, mu, s)), c);
     if (MODE == FORCE_STORE) {
 #pragma unroll
-        for (int a = 0; a < 8; a++) c[a] *= fac;
+        for (int a = 0; a < 8; a++) c[a] = scale1(fac, c[a]);
         stp<T, GF_ST_NT>(FP, V, mu, s, c);
     } else {
         T p[8];
         ldp(FP, V, mu, s, p);
-        const T f = eps * fac;
+        const double f = eps * fac;
 #pragma unroll
-        for (int a = 0; a < 8; a++) p[a] = fma(f, c[a], p[a]);
+        for (int a = 0; a < 8; a++) p[a] = fma1(f, c[a], p[a]);
         stp<T, GF_ST_NT>(FP, V, mu, s, p);
     }
 }
@@ -292,9 +297,9 @@ hipError_t launch_momenta_dot(const Geo& g, const T* P, double* partial, double*
 template <typename T>
 hipError_t launch_dsdu(const Geo& g, const T* U, double coeff, T* D, hipStream_t st) {
     if (g.V % 64 == 0)
-        k_dsdu<T, true><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, (T)coeff, D);
+        k_dsdu<T, true><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, coeff, D);
     else
-        k_dsdu<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, (T)coeff, D);
+        k_dsdu<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, coeff, D);
     return hipGetLastError();
 }
 template <typename T>
@@ -306,23 +311,21 @@ hipError_t launch_force(int mode, const Geo& g, const T* U, T* FP, T* Unew, doub
     switch (mode) {
         case FORCE_STORE:
             if (sh)
-                k_force<T, FORCE_STORE, true><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)fac, (T)eps_u);
+                k_force<T, FORCE_STORE, true><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac, eps_u);
             else
-                k_force<T, FORCE_STORE, false><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)fac, (T)eps_u);
+                k_force<T, FORCE_STORE, false><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac, eps_u);
             break;
         case FORCE_KICK:
             if (sh)
-                k_force<T, FORCE_KICK, true><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)(fac * eps_p), (T)eps_u);
+                k_force<T, FORCE_KICK, true><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac * eps_p, eps_u);
             else
-                k_force<T, FORCE_KICK, false><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)(fac * eps_p), (T)eps_u);
+                k_force<T, FORCE_KICK, false><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac * eps_p, eps_u);
             break;
         case FORCE_KICK_EXP:
             if (sh)
-                k_force<T, FORCE_KICK_EXP, true><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)(fac * eps_p),
-                                                                         (T)eps_u);
+                k_force<T, FORCE_KICK_EXP, true><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac * eps_p, eps_u);
             else
-                k_force<T, FORCE_KICK_EXP, false><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, (T)(fac * eps_p),
-                                                                          (T)eps_u);
+                k_force<T, FORCE_KICK_EXP, false><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac * eps_p, eps_u);
             break;
         default:
             return hipErrorInvalidValue;
@@ -331,16 +334,16 @@ hipError_t launch_force(int mode, const Geo& g, const T* U, T* FP, T* Unew, doub
 }
 template <typename T>
 hipError_t launch_exp_update(const Geo& g, const T* Uin, T* Uout, const T* P, double eps, hipStream_t st) {
-    k_exp_update<T><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, Uout, P, (T)eps);
+    k_exp_update<T><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, Uout, P, eps);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_force_from_deriv(int mode, const Geo& g, const T* U, const T* D, T* FP, double eps,
                                    hipStream_t st) {
     if (mode == FORCE_STORE)
-        k_force_from_deriv<T, FORCE_STORE><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, D, FP, (T)eps);
+        k_force_from_deriv<T, FORCE_STORE><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, D, FP, eps);
     else if (mode == FORCE_KICK)
-        k_force_from_deriv<T, FORCE_KICK><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, D, FP, (T)eps);
+        k_force_from_deriv<T, FORCE_KICK><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, D, FP, eps);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

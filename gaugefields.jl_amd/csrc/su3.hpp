@@ -21,6 +21,22 @@ struct M3 {
 };
 
 // ---------------------------------------------------------------------------
+// Scalars (step sizes, rho, coupling factors, 1/3, 1/sqrt(3)) are applied in double and the
+// product rounded once to T.  A float-rounded scalar would scale every element of a field by
+// the same relative error (float(-1/3) is 3e-8 too large): a systematic force-scale bias of
+// that size makes an FP32-bulk trajectory drift from energy conservation by ~1e-8 of the
+// action change, visible in dH at 32^4 and above.  For T = double these are a*x and fma.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T scale1(double a, T x) {
+    return (T)(a * (double)x);
+}
+template <typename T>
+__device__ __forceinline__ T fma1(double a, T x, T y) {
+    return (T)fma(a, (double)x, (double)y);
+}
+
+// ---------------------------------------------------------------------------
 // complex helpers
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -256,7 +272,6 @@ __device__ __forceinline__ void reunitarize(M3<T>& u) {
 // coefficients of TA(v) = (v - v^dag)/2 - tr(v - v^dag)/6
 template <typename T>
 __device__ __forceinline__ void ta_coeffs(const M3<T>& v, T c[8]) {
-    const T s3 = T(GF_SR3I);
     c[0] = v.m[1].i + v.m[3].i;
     c[1] = v.m[1].r - v.m[3].r;
     c[2] = v.m[0].i - v.m[4].i;
@@ -264,14 +279,14 @@ __device__ __forceinline__ void ta_coeffs(const M3<T>& v, T c[8]) {
     c[4] = v.m[2].r - v.m[6].r;
     c[5] = v.m[5].i + v.m[7].i;
     c[6] = v.m[5].r - v.m[7].r;
-    c[7] = s3 * (v.m[0].i + v.m[4].i - T(2) * v.m[8].i);
+    c[7] = scale1(GF_SR3I, v.m[0].i + v.m[4].i - T(2) * v.m[8].i);
 }
 
 // TA(v) as a matrix
 template <typename T>
 __device__ __forceinline__ M3<T> ta_mat(const M3<T>& v) {
     M3<T> r;
-    T tri = (v.m[0].i + v.m[4].i + v.m[8].i) * T(1.0 / 3.0);
+    const T tri = (T)((double)(v.m[0].i + v.m[4].i + v.m[8].i) / 3.0);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         r.m[4 * i].r = T(0);
@@ -324,11 +339,11 @@ struct Herm {
 // sum_a c_a lambda_a / 2 (the Hermitian part of a traceless anti-Hermitian i sum_a c_a lambda_a / 2)
 template <typename T>
 __device__ __forceinline__ Herm<T> herm_of_coeffs(const T c[8]) {
-    const T s3 = T(GF_SR3I);
+    const T s8 = scale1(GF_SR3I, c[7]);
     Herm<T> h;
-    h.d[0] = T(0.5) * (c[2] + s3 * c[7]);
-    h.d[1] = T(0.5) * (-c[2] + s3 * c[7]);
-    h.d[2] = -s3 * c[7];
+    h.d[0] = T(0.5) * (c[2] + s8);
+    h.d[1] = T(0.5) * (-c[2] + s8);
+    h.d[2] = -s8;
     h.u[0] = {T(0.5) * c[0], T(-0.5) * c[1]};
     h.u[1] = {T(0.5) * c[3], T(-0.5) * c[4]};
     h.u[2] = {T(0.5) * c[5], T(-0.5) * c[6]};
@@ -820,12 +835,12 @@ __device__ __forceinline__ M3<T> round_link(const M3<double>& d) {
 
 // exp(i t sum_a c_a lambda_a / 2) U
 template <typename T>
-__device__ __forceinline__ M3<T> exp_coeffs_times(const T c[8], T t, const M3<T>& u) {
+__device__ __forceinline__ M3<T> exp_coeffs_times(const T c[8], double t, const M3<T>& u) {
     if constexpr (std::is_same<T, float>::value) {
         double cd[8];
 #pragma unroll
         for (int a = 0; a < 8; a++) cd[a] = (double)c[a];
-        M3<double> r = mul(expi_herm(herm_from_coeffs(cd, (double)t)), to_f64(u));
+        M3<double> r = mul(expi_herm(herm_from_coeffs(cd, t)), to_f64(u));
         reunitarize_f64(r);
         return round_link<T>(r);
     } else {
@@ -833,6 +848,12 @@ __device__ __forceinline__ M3<T> exp_coeffs_times(const T c[8], T t, const M3<T>
     }
 }
 
+// exp(Q) U for a traceless anti-Hermitian Q given in double (FP32 links: rounded once)
+__device__ __forceinline__ M3<float> exp_antiherm_times(const M3<double>& Q, const M3<float>& u) {
+    M3<double> r = mul(exp_antiherm(Q), to_f64(u));
+    reunitarize_f64(r);
+    return round_link<float>(r);
+}
 // exp(Q) U for a traceless anti-Hermitian Q
 template <typename T>
 __device__ __forceinline__ M3<T> exp_antiherm_times(const M3<T>& Q, const M3<T>& u) {
