@@ -89,10 +89,9 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
     int s;
     if (!stencil_site<SHARED>(g, sm, s)) return;
     const unsigned V = g.V;
-    M3<T> C;
+    M3<T> C;  // recomputed C; a stored C is loaded where it is used instead of kept live
     R6<T> own;
     if (STORED) {
-        C = ldm(Cst, V, mu, s);
         own = ldr(Uin, V, mu, s);
     } else {  // the same staple evaluation (and rounding) as the forward kernel
         int x[4];
@@ -102,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
     M3<T> E, L;
     {
         const M3<T> u = full_of(own);
-        exp_dexp(ta_mat(mul_nd(C, u)), mul(u, ldm(Dout, V, mu, s)), E, L);
+        exp_dexp(ta_mat(mul_nd(STORED ? ldm(Cst, V, mu, s) : C, u)), mul(u, ldm(Dout, V, mu, s)), E, L);
     }
     const M3<T> Lam = ta_mat(L);
     {
@@ -114,7 +113,7 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
     int s1 = s;
     asm volatile("" : "+v"(s1));
     M3<T> D = mul(ldm(Dout, V, mu, s1), E);
-    acc_dag(D, mul(Lam, C));
+    acc_dag(D, mul(Lam, STORED ? ldm(Cst, V, mu, s1) : C));
     stm<T, GF_ST_NT>(Din, V, mu, s, D);
 }
 

@@ -773,17 +773,6 @@ __device__ __forceinline__ void exp_dexp(const M3<T>& Q, const M3<T>& X, M3<T>& 
     ch_coeffs<true>(c0, c1, f, b1, b2);
     const T f0r = T(f[0].r), f0i = T(f[0].i), f1r = T(f[1].r), f1i = T(f[1].i);
     const T f2r = T(f[2].r), f2i = T(f[2].i);
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-        E.m[k].r = f1r * H.m[k].r - f1i * H.m[k].i + f2r * H2.m[k].r - f2i * H2.m[k].i;
-        E.m[k].i = f1r * H.m[k].i + f1i * H.m[k].r + f2r * H2.m[k].i + f2i * H2.m[k].r;
-    }
-    E.m[0].r += f0r;
-    E.m[0].i += f0i;
-    E.m[4].r += f0r;
-    E.m[4].i += f0i;
-    E.m[8].r += f0r;
-    E.m[8].i += f0i;
     dc t0 = {0.0, 0.0}, t1 = {0.0, 0.0}, t2 = {0.0, 0.0};
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -814,6 +803,18 @@ __device__ __forceinline__ void exp_dexp(const M3<T>& Q, const M3<T>& X, M3<T>& 
         L.m[k].r = im;
         L.m[k].i = -r;
     }
+    // exp(Q) last: it is not needed while L is formed
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        E.m[k].r = f1r * H.m[k].r - f1i * H.m[k].i + f2r * H2.m[k].r - f2i * H2.m[k].i;
+        E.m[k].i = f1r * H.m[k].i + f1i * H.m[k].r + f2r * H2.m[k].i + f2i * H2.m[k].r;
+    }
+    E.m[0].r += f0r;
+    E.m[0].i += f0i;
+    E.m[4].r += f0r;
+    E.m[4].i += f0i;
+    E.m[8].r += f0r;
+    E.m[8].i += f0i;
 }
 
 
