@@ -128,6 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
 // Z = U_a(y-b)^dag U_b(y-b),
 //   upper = X Lam_a(y+b) U_b(y)^dag + S Lam_b(y) - Lam_b(y+a) S
 //   lower = U_b(y+a-b)^dag [U_a(y-b)^dag (Lam_a(y-b) - Lam_b(y-b)) U_b(y-b) + Lam_b(y+a-b) Z]
+//         = U_b(y+a-b)^dag [U_a(y-b)^dag (Lam_a(y-b) - Lam_b(y-b)) + Lam_b(y+a-b) U_a(y-b)^dag] U_b(y-b)
 // Six links and six 8-real Lam per plane instead of ten links and six 18-real G.  Lam = i H with
 // H Hermitian (its 8 coefficients): products with Lam are i times Hermitian-structured products.
 // ---------------------------------------------------------------------------
@@ -137,19 +138,71 @@ __device__ __forceinline__ Herm<T> herm_at(const T* __restrict__ Lc, unsigned V,
     ldp(Lc, V, mu, s, c);
     return herm_of_coeffs(c);
 }
-// c += i m
-template <typename T>
-__device__ __forceinline__ void acc_i(M3<T>& c, const M3<T>& m) {
+// c += SG i m h  (SG = +1 or -1), each product element accumulated as it is formed
+template <typename T, int SG>
+__device__ __forceinline__ void acc_i_mh(M3<T>& c, const M3<T>& m, const Herm<T>& h) {
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        c.m[k].r -= m.m[k].i;
-        c.m[k].i += m.m[k].r;
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cx<T> t;
+            t.r = m.m[3 * i + j].r * h.d[j];
+            t.i = m.m[3 * i + j].i * h.d[j];
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (k != j) cmac(t, m.m[3 * i + k], herm_off(h, k, j));
+            c.m[3 * i + j].r -= T(SG) * t.i;
+            c.m[3 * i + j].i += T(SG) * t.r;
+        }
+}
+// c += SG i h m
+template <typename T, int SG>
+__device__ __forceinline__ void acc_i_hm(M3<T>& c, const Herm<T>& h, const M3<T>& m) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cx<T> t;
+            t.r = h.d[i] * m.m[3 * i + j].r;
+            t.i = h.d[i] * m.m[3 * i + j].i;
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (k != i) cmac(t, herm_off(h, i, k), m.m[3 * k + j]);
+            c.m[3 * i + j].r -= T(SG) * t.i;
+            c.m[3 * i + j].i += T(SG) * t.r;
+        }
+}
+// c += h m^dag
+template <typename T>
+__device__ __forceinline__ void acc_hd(M3<T>& c, const Herm<T>& h, const M3<T>& m) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cx<T>& t = c.m[3 * i + j];
+            t.r = fma(h.d[i], m.m[3 * j + i].r, t.r);
+            t.i = fma(-h.d[i], m.m[3 * j + i].i, t.i);
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (k != i) cmac_nc(t, herm_off(h, i, k), m.m[3 * j + k]);
+        }
+}
+// the two stored rows of an SU(3) matrix held whole
+template <typename T>
+__device__ __forceinline__ R6<T> rows_of(const M3<T>& m) {
+    R6<T> r;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        r.v[k].x = m.m[k].r;
+        r.v[k].y = m.m[k].i;
     }
+    return r;
 }
 
+// At most 168 VGPRs (FP64: 166, no spills), so that three waves fit per SIMD.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* __restrict__ U,
-                                                               const T* __restrict__ Lc, T* __restrict__ Din, double rho) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_stout_back_stencil(
+    Geo g, const T* __restrict__ U, const T* __restrict__ Lc, T* __restrict__ Din, double rho) {
     const SiteMu sm = site_mu_index(g);
     const int y = sm.s, a = sm.mu;
     if (y >= g.V) return;
@@ -166,39 +219,29 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_stencil(Geo g, const T* _
         const int ymb0 = y + dmb, ypamb0 = ypa + dmb;
         // each group loads its links through its own opaque index copies, so the compiler
         // issues them group by group instead of hoisting all of them
-        {  // upper: i [X H_a(y+b)] U_b(y)^dag + i [S H_b(y) - H_b(y+a) S]
+        {  // upper: i X H_a(y+b) U_b(y)^dag + i [S H_b(y) - H_b(y+a) S],  S = X U_b(y)^dag
             int y0 = y, y1 = ypa, ypb = ypb0;
             asm volatile("" : "+v"(y0), "+v"(y1), "+v"(ypb));
             const M3<T> X = su3_mul_nd(ldr(U, V, b, y1), ldu(U, V, a, ypb));
             const M3<T> ub = ldu(U, V, b, y0);
             acc_mul_nd(acc, times_i(mul_mh(X, herm_at(Lc, V, a, ypb))), ub);
-            const M3<T> S = mul_nd(X, ub);
-            M3<T> t = mul_mh(S, herm_at(Lc, V, b, y0));
-            const M3<T> t2 = mul_hm(herm_at(Lc, V, b, y1), S);
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                t.m[k].r -= t2.m[k].r;
-                t.m[k].i -= t2.m[k].i;
-            }
-            acc_i(acc, t);
+            const M3<T> S = su3_mul_nd(rows_of(X), ub);
+            acc_i_mh<T, 1>(acc, S, herm_at(Lc, V, b, y0));
+            acc_i_hm<T, -1>(acc, herm_at(Lc, V, b, y1), S);
         }
-        {  // lower: U_b(y+a-b)^dag i [U_a(y-b)^dag H_d U_b(y-b) + H_b(y+a-b) Z], H_d = H_a(y-b) - H_b(y-b)
+        {  // lower: U_b(y+a-b)^dag i [U_a(y-b)^dag H_d + H_b(y+a-b) U_a(y-b)^dag] U_b(y-b),
+           // H_d = H_a(y-b) - H_b(y-b)
             int ymb = ymb0, ypamb = ypamb0;
             asm volatile("" : "+v"(ymb), "+v"(ypamb));
-            const M3<T> ua = ldu(U, V, a, ymb), ub = ldu(U, V, b, ymb);
+            const M3<T> ua = ldu(U, V, a, ymb);
             T ca[8], cb[8];
             ldp(Lc, V, a, ymb, ca);
             ldp(Lc, V, b, ymb, cb);
 #pragma unroll
             for (int k = 0; k < 8; k++) ca[k] -= cb[k];
-            M3<T> t = mul(mul_dh(ua, herm_of_coeffs(ca)), ub);
-            const M3<T> t2 = mul_hm(herm_at(Lc, V, b, ypamb), mul_dn(ua, ub));
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                t.m[k].r += t2.m[k].r;
-                t.m[k].i += t2.m[k].i;
-            }
-            acc_mul_dn(acc, ldu(U, V, b, ypamb), times_i(t));
+            M3<T> A = mul_dh(ua, herm_of_coeffs(ca));
+            acc_hd(A, herm_at(Lc, V, b, ypamb), ua);
+            acc_mul_dn(acc, ldu(U, V, b, ypamb), times_i(mul(A, ldu(U, V, b, ymb))));
         }
     }
     M3<T> dn = ldm(Din, V, a, y);
