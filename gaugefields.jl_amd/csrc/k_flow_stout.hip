@@ -10,11 +10,13 @@ namespace gf {
 //   Wout_mu = exp(A_mu) W_mu               (exp_aF_U!, AbstractGaugefields.jl:2810-2841)
 // stages (alpha, beta) = (-eps/4, 0), (-8eps/9, -17/9), (-3eps/4, -1)
 // ---------------------------------------------------------------------------
-// SHARED: upper-staple products shared between directions through LDS (V % 64 == 0)
+// SHARED: upper-staple plaquettes shared between directions through LDS (ta_force_shared;
+// V % 64 == 0)
 template <typename T, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restrict__ W, T* __restrict__ Wout,
                                                        T* __restrict__ A, double alpha, double beta) {
-    __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
+    __shared__ T2<T> ol[SHARED ? 4 : 1][6][64];
+    __shared__ T2<T> tp[SHARED ? 7 : 1][4][64];
     const SiteMu sm = site_mu_index(g);
     const int mu = sm.mu;
     int s;
@@ -23,10 +25,14 @@ __global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restric
     decode(g, s, x);
     const unsigned V = g.V;
     R6<T> own;
-    const M3<T> sig = staple_sum_own<T, SHARED>(g, W, V, s, x, mu, own, xs);
-    const M3<T> w = full_of(own);
     T c[8];
-    ta_coeffs(mul(w, sig), c);
+    if constexpr (SHARED) {
+        ta_force_shared(g, W, V, s, x, mu, own, ol, tp, c);
+    } else {
+        const M3<T> sig = staple_sigma_own(g, W, V, s, x, mu, own);
+        ta_coeffs(mul(full_of(own), sig), c);
+    }
+    const M3<T> w = full_of(own);
     if (beta != T(0)) {
         T a[8];
         ldp(A, V, mu, s, a);
@@ -186,17 +192,6 @@ __device__ __forceinline__ void acc_hd(M3<T>& c, const Herm<T>& h, const M3<T>& 
             for (int k = 0; k < 3; k++)
                 if (k != i) cmac_nc(t, herm_off(h, i, k), m.m[3 * j + k]);
         }
-}
-// the two stored rows of an SU(3) matrix held whole
-template <typename T>
-__device__ __forceinline__ R6<T> rows_of(const M3<T>& m) {
-    R6<T> r;
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-        r.v[k].x = m.m[k].r;
-        r.v[k].y = m.m[k].i;
-    }
-    return r;
 }
 
 // At most 168 VGPRs (FP64: 166, no spills), so that three waves fit per SIMD.

@@ -94,11 +94,13 @@ __global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U,
 //   FORCE_STORE    : FP = F
 //   FORCE_KICK     : FP += eps_p F                    (update_momenta!, :539-551)
 //   FORCE_KICK_EXP : FP += eps_p F; Unew = exp(eps_u FP) U   (then update_gaugefields!, :513-531)
-// SHARED: upper-staple products shared between directions through LDS (V % 64 == 0)
+// SHARED: upper-staple plaquettes shared between directions through LDS (ta_force_shared;
+// V % 64 == 0)
 template <typename T, int MODE, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U, T* __restrict__ FP,
                                                   T* __restrict__ Unew, double fac, double eps_u) {
-    __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
+    __shared__ T2<T> ol[SHARED ? 4 : 1][6][64];
+    __shared__ T2<T> tp[SHARED ? 7 : 1][4][64];
     const SiteMu sm = site_mu_index(g);
     const int mu = sm.mu;
     int s;
@@ -107,10 +109,14 @@ __global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U
     decode(g, s, x);
     const unsigned V = g.V;
     R6<T> own;
-    const M3<T> sig = staple_sum_own<T, SHARED>(g, U, V, s, x, mu, own, xs);
-    const M3<T> umu = full_of(own);
     T c[8];
-    ta_coeffs(mul(umu, sig), c);
+    if constexpr (SHARED) {
+        ta_force_shared(g, U, V, s, x, mu, own, ol, tp, c);
+    } else {
+        const M3<T> sig = staple_sigma_own(g, U, V, s, x, mu, own);
+        ta_coeffs(mul(full_of(own), sig), c);
+    }
+    const M3<T> umu = full_of(own);
     if (MODE == FORCE_STORE) {
 #pragma unroll
         for (int a = 0; a < 8; a++) c[a] = scale1(fac, c[a]);

@@ -228,6 +228,17 @@ __device__ __forceinline__ cx<T> rc(const R6<T>& r, int k) {
     c.i = r.v[k].y;
     return c;
 }
+// the two stored rows of an SU(3) matrix held whole
+template <typename T>
+__device__ __forceinline__ R6<T> rows_of(const M3<T>& m) {
+    R6<T> r;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        r.v[k].x = m.m[k].r;
+        r.v[k].y = m.m[k].i;
+    }
+    return r;
+}
 // products whose result is SU(3): rows 0 and 1 from the two stored rows of a, row 2 rebuilt
 template <typename T>
 __device__ __forceinline__ M3<T> su3_mul_nd(const R6<T>& a, const M3<T>& b) {  // a b^dag
@@ -346,7 +357,8 @@ __device__ __forceinline__ void stp(T* __restrict__ p, unsigned V, int mu, unsig
 __device__ __forceinline__ int sel4(int i, int a0, int a1, int a2, int a3) {
     return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
 }
-template <typename T, bool OWN>
+// LOWER: the lower staples only (links 3..5 of every block) and, with OWN, U_mu(x)
+template <typename T, bool OWN, bool LOWER = false>
 __device__ __forceinline__ M3<T> staple_sigma_pipe(const Geo& g, const T* __restrict__ U, unsigned V, int s,
                                                    const int x[4], int mu, R6<T>* own) {
     int fw[4], bw[4];
@@ -373,14 +385,16 @@ __device__ __forceinline__ M3<T> staple_sigma_pipe(const Geo& g, const T* __rest
         if (r == 4) site = s + fmu + bnu;
         return ldr(U, V, dir, (unsigned)site);
     };
-    constexpr int NL = OWN ? 19 : 18;
+    constexpr int NL = LOWER ? (OWN ? 10 : 9) : (OWN ? 19 : 18);
+    auto slot = [](int i) { return LOWER ? (i < 9 ? 6 * (i / 3) + 3 + i % 3 : 18) : i; };
     M3<T> sig = zero3<T>();
     M3<T> X;
-    R6<T> A, nxt = issue(0);
+    R6<T> A, nxt = issue(slot(0));
 #pragma unroll
-    for (int k = 0; k < NL; k++) {
+    for (int i = 0; i < NL; i++) {
+        const int k = slot(i);
         const R6<T> cur = nxt;
-        if (k + 1 < NL) nxt = issue(k + 1);
+        if (i + 1 < NL) nxt = issue(slot(i + 1));
         __builtin_amdgcn_sched_barrier(0);
         if (k == 18) {
             *own = cur;
@@ -529,6 +543,84 @@ __device__ __forceinline__ M3<T> staple_sigma_shared(const Geo& g, const T* __re
         }
     }
     return sig;
+}
+
+// ---------------------------------------------------------------------------
+// TA(U_mu(x) Sigma_mu(x)) (8 su(3) coefficients, ta_coeffs) with the upper staples taken as
+// whole plaquettes shared between directions.  U_mu(x) X_mu,nu U_nu(x)^dag is the plaquette
+// P_mu,nu(x), and P_nu,mu = P_mu,nu^dag, so TA(P_nu,mu) = -TA(P_mu,nu).  Each unordered pair's
+// plaquette is formed once (by the waves and in the slots of staple_sigma_shared), only its 8
+// TA coefficients go to LDS, and the links U_nu(x) reach the other waves through LDS as well:
+// per wave 14 link loads (88 vector-memory loads with the momenta) instead of 17 (106).
+// Order: lower staples, U_mu(x) (-> LDS, TA of U_mu times the lower staples), the pairs.
+// TA is linear, so the result is TA(U_mu Sigma_mu) up to rounding.  Two workgroup barriers
+// (every lane must reach them: V a multiple of 64).
+// LDS: ol[4][6][64] links and tp[7][4][64] coefficient pairs (52 KB FP64, 26 KB FP32).
+template <typename T>
+using PairShare = T2<T> (*)[4][64];
+template <typename T>
+__device__ __forceinline__ void ta_force_shared(const Geo& g, const T* __restrict__ U, unsigned V, int s,
+                                                const int x[4], int mu, R6<T>& own, XShare<T> ol,
+                                                PairShare<T> tp, T c[8]) {
+    const int lane = (int)(threadIdx.x & 63u);
+    int fw[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) fw[d] = dlt(g, x, d, 1);
+    // pairs formed by this wave: w0 (0,1),(0,3); w1 (1,2),(1,3); w2 (0,2); w3 (2,3); waves 2 and 3
+    // form (2,3) a second time into the unused slot 6 (same branch-free sequence for all)
+    const int pa1 = mu == 2 ? 0 : (mu == 3 ? 2 : mu);
+    const int pb1 = mu == 0 ? 1 : (mu == 1 ? 2 : (mu == 2 ? 2 : 3));
+    const int pa2 = mu < 2 ? mu : 2;
+    const int pb2 = 3;
+    const int p1 = pair_index(pa1, pb1), p2 = mu < 2 ? pair_index(pa2, pb2) : 6;
+    // TA coefficients of P_a,b = U_a(x) X U_b(x)^dag into slot p
+    auto plaq = [&](int a, int b, const M3<T>& X, int p) {
+        R6<T> ra, rb;
+#pragma unroll
+        for (int e = 0; e < 6; e++) {
+            ra.v[e] = ol[a][e][lane];
+            rb.v[e] = ol[b][e][lane];
+        }
+        T q[8];
+        ta_coeffs(su3_mul_nd(rows_of(su3_mul(ra, X)), full_of(rb)), q);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            T2<T> v;
+            v.x = q[2 * e];
+            v.y = q[2 * e + 1];
+            tp[p][e][lane] = v;
+        }
+    };
+    const M3<T> sig = staple_sigma_pipe<T, true, true>(g, U, V, s, x, mu, &own);
+#pragma unroll
+    for (int e = 0; e < 6; e++) ol[mu][e][lane] = own.v[e];
+    ta_coeffs(mul(full_of(own), sig), c);
+    // pin c here: otherwise the staple arithmetic is sunk past the pairs to its use, keeping
+    // every loaded staple link live across them
+    asm volatile("" : "+v"(c[0]), "+v"(c[1]), "+v"(c[2]), "+v"(c[3]), "+v"(c[4]), "+v"(c[5]), "+v"(c[6]),
+                 "+v"(c[7]));
+    // the pairs: a loop of its own, so that its loads are not hoisted into the staples above
+#pragma unroll 1
+    for (int q = 0; q < 2; q++) {
+        const int a = q == 0 ? pa1 : pa2, b = q == 0 ? pb1 : pb2;
+        const int fa = sel4(a, fw[0], fw[1], fw[2], fw[3]), fb = sel4(b, fw[0], fw[1], fw[2], fw[3]);
+        const M3<T> Xp = su3_mul_nd(ldr(U, V, b, (unsigned)(s + fa)), ldu(U, V, a, (unsigned)(s + fb)));
+        if (q == 0) __syncthreads();  // every wave's U(x) is in LDS
+        plaq(a, b, Xp, q == 0 ? p1 : p2);
+    }
+    __syncthreads();  // every pair's coefficients are in LDS
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int nu = j + (j >= mu ? 1 : 0);
+        const int p = pair_index(mu < nu ? mu : nu, mu < nu ? nu : mu);
+        const T sg = mu < nu ? T(1) : T(-1);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const T2<T> v = tp[p][e][lane];
+            c[2 * e] = fma(sg, v.x, c[2 * e]);
+            c[2 * e + 1] = fma(sg, v.y, c[2 * e + 1]);
+        }
+    }
 }
 
 // staple_sigma_own or, when SHARED, staple_sigma_shared (xs: the kernel's LDS array)
