@@ -10,23 +10,24 @@ namespace gf {
 //   Wout_mu = exp(A_mu) W_mu               (exp_aF_U!, AbstractGaugefields.jl:2810-2841)
 // stages (alpha, beta) = (-eps/4, 0), (-8eps/9, -17/9), (-3eps/4, -1)
 // ---------------------------------------------------------------------------
-// SHARED: upper-staple plaquettes shared between directions through LDS (ta_force_shared;
-// V % 64 == 0)
-template <typename T, bool SHARED>
-__global__ __launch_bounds__(kBlock) void k_flow_stage(Geo g, const T* __restrict__ W, T* __restrict__ Wout,
+// ST: stencil form as in k_force (stencil_form).  At most 168 VGPRs: three waves per SIMD.
+template <typename T, int ST>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_flow_stage(Geo g, const T* __restrict__ W, T* __restrict__ Wout,
                                                        T* __restrict__ A, double alpha, double beta) {
-    __shared__ T2<T> ol[SHARED ? 4 : 1][6][64];
-    __shared__ T2<T> tp[SHARED ? 7 : 1][4][64];
+    __shared__ T2<T> ol[ST ? 4 : 1][6][64];
+    __shared__ T2<T> tp[ST ? 7 : 1][4][64];
     const SiteMu sm = site_mu_index(g);
     const int mu = sm.mu;
     int s;
-    if (!stencil_site<SHARED>(g, sm, s)) return;
+    if (!stencil_site<(ST > 0)>(g, sm, s)) return;
     int x[4];
     decode(g, s, x);
     const unsigned V = g.V;
     R6<T> own;
     T c[8];
-    if constexpr (SHARED) {
+    if constexpr (ST == ST_XROW) {
+        ta_force_xrow(g, W, V, s, x, mu, own, ol, tp, c);
+    } else if constexpr (ST == ST_SHARED) {
         ta_force_shared(g, W, V, s, x, mu, own, ol, tp, c);
     } else {
         const M3<T> sig = staple_sigma_own(g, W, V, s, x, mu, own);
@@ -251,10 +252,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 // ---------------------------------------------------------------------------
 template <typename T>
 hipError_t launch_flow_stage(const Geo& g, const T* W, T* Wout, T* A, double alpha, double beta, hipStream_t st) {
-    if (g.V % 64 == 0)
-        k_flow_stage<T, true><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, W, Wout, A, alpha, beta);
+    const int form = stencil_form(g), nb = grid_sites_mu(g.V);
+    if (form == ST_XROW)
+        k_flow_stage<T, ST_XROW><<<nb, kBlock, 0, st>>>(g, W, Wout, A, alpha, beta);
+    else if (form == ST_SHARED)
+        k_flow_stage<T, ST_SHARED><<<nb, kBlock, 0, st>>>(g, W, Wout, A, alpha, beta);
     else
-        k_flow_stage<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, W, Wout, A, alpha, beta);
+        k_flow_stage<T, ST_OWN><<<nb, kBlock, 0, st>>>(g, W, Wout, A, alpha, beta);
     return hipGetLastError();
 }
 template <typename T>

@@ -94,23 +94,26 @@ __global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U,
 //   FORCE_STORE    : FP = F
 //   FORCE_KICK     : FP += eps_p F                    (update_momenta!, :539-551)
 //   FORCE_KICK_EXP : FP += eps_p F; Unew = exp(eps_u FP) U   (then update_gaugefields!, :513-531)
-// SHARED: upper-staple plaquettes shared between directions through LDS (ta_force_shared;
-// V % 64 == 0)
-template <typename T, int MODE, bool SHARED>
-__global__ __launch_bounds__(kBlock) void k_force(Geo g, const T* __restrict__ U, T* __restrict__ FP,
+// ST (stencil_form): ST_OWN per-thread staples; ST_SHARED upper-staple plaquettes shared between
+// directions through LDS (ta_force_shared, V % 64 == 0); ST_XROW the same with the x-neighbour
+// links from LDS as well (ta_force_xrow, NX divides 64).  At most 168 VGPRs: three waves per SIMD.
+template <typename T, int MODE, int ST>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_force(Geo g, const T* __restrict__ U, T* __restrict__ FP,
                                                   T* __restrict__ Unew, double fac, double eps_u) {
-    __shared__ T2<T> ol[SHARED ? 4 : 1][6][64];
-    __shared__ T2<T> tp[SHARED ? 7 : 1][4][64];
+    __shared__ T2<T> ol[ST ? 4 : 1][6][64];
+    __shared__ T2<T> tp[ST ? 7 : 1][4][64];
     const SiteMu sm = site_mu_index(g);
     const int mu = sm.mu;
     int s;
-    if (!stencil_site<SHARED>(g, sm, s)) return;
+    if (!stencil_site<(ST > 0)>(g, sm, s)) return;
     int x[4];
     decode(g, s, x);
     const unsigned V = g.V;
     R6<T> own;
     T c[8];
-    if constexpr (SHARED) {
+    if constexpr (ST == ST_XROW) {
+        ta_force_xrow(g, U, V, s, x, mu, own, ol, tp, c);
+    } else if constexpr (ST == ST_SHARED) {
         ta_force_shared(g, U, V, s, x, mu, own, ol, tp, c);
     } else {
         const M3<T> sig = staple_sigma_own(g, U, V, s, x, mu, own);
@@ -308,30 +311,31 @@ hipError_t launch_dsdu(const Geo& g, const T* U, double coeff, T* D, hipStream_t
         k_dsdu<T, false><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, coeff, D);
     return hipGetLastError();
 }
+template <typename T, int MODE>
+void launch_force_st(int form, int nb, hipStream_t st, const Geo& g, const T* U, T* FP, T* Unew, double fac,
+                     double eps_u) {
+    if (form == ST_XROW)
+        k_force<T, MODE, ST_XROW><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac, eps_u);
+    else if (form == ST_SHARED)
+        k_force<T, MODE, ST_SHARED><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac, eps_u);
+    else
+        k_force<T, MODE, ST_OWN><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac, eps_u);
+}
 template <typename T>
 hipError_t launch_force(int mode, const Geo& g, const T* U, T* FP, T* Unew, double coeff, double eps_p,
                         double eps_u, hipStream_t st) {
     const int nb = grid_sites_mu(g.V);
     const double fac = -coeff / 3.0;
-    const bool sh = g.V % 64 == 0;
+    const int form = stencil_form(g);
     switch (mode) {
         case FORCE_STORE:
-            if (sh)
-                k_force<T, FORCE_STORE, true><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac, eps_u);
-            else
-                k_force<T, FORCE_STORE, false><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac, eps_u);
+            launch_force_st<T, FORCE_STORE>(form, nb, st, g, U, FP, Unew, fac, eps_u);
             break;
         case FORCE_KICK:
-            if (sh)
-                k_force<T, FORCE_KICK, true><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac * eps_p, eps_u);
-            else
-                k_force<T, FORCE_KICK, false><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac * eps_p, eps_u);
+            launch_force_st<T, FORCE_KICK>(form, nb, st, g, U, FP, Unew, fac * eps_p, eps_u);
             break;
         case FORCE_KICK_EXP:
-            if (sh)
-                k_force<T, FORCE_KICK_EXP, true><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac * eps_p, eps_u);
-            else
-                k_force<T, FORCE_KICK_EXP, false><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac * eps_p, eps_u);
+            launch_force_st<T, FORCE_KICK_EXP>(form, nb, st, g, U, FP, Unew, fac * eps_p, eps_u);
             break;
         default:
             return hipErrorInvalidValue;

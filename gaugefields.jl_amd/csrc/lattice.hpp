@@ -357,8 +357,8 @@ __device__ __forceinline__ void stp(T* __restrict__ p, unsigned V, int mu, unsig
 __device__ __forceinline__ int sel4(int i, int a0, int a1, int a2, int a3) {
     return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
 }
-// LOWER: the lower staples only (links 3..5 of every block) and, with OWN, U_mu(x)
-template <typename T, bool OWN, bool LOWER = false>
+// LOWER: the lower staples only (links 3..5 of blocks J0..2) and, with OWN, U_mu(x)
+template <typename T, bool OWN, bool LOWER = false, int J0 = 0>
 __device__ __forceinline__ M3<T> staple_sigma_pipe(const Geo& g, const T* __restrict__ U, unsigned V, int s,
                                                    const int x[4], int mu, R6<T>* own) {
     int fw[4], bw[4];
@@ -385,8 +385,9 @@ __device__ __forceinline__ M3<T> staple_sigma_pipe(const Geo& g, const T* __rest
         if (r == 4) site = s + fmu + bnu;
         return ldr(U, V, dir, (unsigned)site);
     };
-    constexpr int NL = LOWER ? (OWN ? 10 : 9) : (OWN ? 19 : 18);
-    auto slot = [](int i) { return LOWER ? (i < 9 ? 6 * (i / 3) + 3 + i % 3 : 18) : i; };
+    constexpr int NLOW = 3 * (3 - J0);
+    constexpr int NL = LOWER ? (OWN ? NLOW + 1 : NLOW) : (OWN ? 19 : 18);
+    auto slot = [](int i) { return LOWER ? (i < NLOW ? 6 * (i / 3 + J0) + 3 + i % 3 : 18) : i; };
     M3<T> sig = zero3<T>();
     M3<T> X;
     R6<T> A, nxt = issue(slot(0));
@@ -623,6 +624,85 @@ __device__ __forceinline__ void ta_force_shared(const Geo& g, const T* __restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// ta_force_shared for workgroups of whole x-rows (NX divides 64, V a multiple of 64): the
+// x-neighbours x +- 1 of every site are sites of the same workgroup, so the links at them come
+// from the U(x) copies in LDS instead of memory: U_mu(x-1), U_0(x-1) of the nu = 0 lower staple
+// and U_b(x+1) of the pairs (0, b).  Pairs: w0 (0,3); w1 (1,2), (0,1); w2 (2,3), (0,2); w3 (1,3)
+// (link loads per wave 12, 11, 11, 10 instead of 14).
+// LDS: ol[4][6][64] links and tp[6][4][64] coefficient pairs.
+__device__ __forceinline__ int xlane(int lane, int nxm1, int d) { return (lane & ~nxm1) | ((lane + d) & nxm1); }
+template <typename T>
+__device__ __forceinline__ void ta_force_xrow(const Geo& g, const T* __restrict__ U, unsigned V, int s,
+                                              const int x[4], int mu, R6<T>& own, XShare<T> ol,
+                                              PairShare<T> tp, T c[8]) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const int nxm1 = g.n[0] - 1;
+    const int lxm = xlane(lane, nxm1, -1), lxp = xlane(lane, nxm1, 1);
+    int fw[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) fw[d] = dlt(g, x, d, 1);
+    auto lds_link = [&](int d, int l) {
+        R6<T> r;
+#pragma unroll
+        for (int e = 0; e < 6; e++) r.v[e] = ol[d][e][l];
+        return r;
+    };
+    {
+        const R6<T> o = ldr(U, V, mu, (unsigned)s);
+#pragma unroll
+        for (int e = 0; e < 6; e++) ol[mu][e][lane] = o.v[e];
+    }
+    __syncthreads();  // every wave's U(x) is in LDS
+    M3<T> sig;
+    if (mu == 0) {
+        sig = staple_sigma_pipe<T, false, true, 0>(g, U, V, s, x, mu, nullptr);
+    } else {
+        sig = staple_sigma_pipe<T, false, true, 1>(g, U, V, s, x, mu, nullptr);
+        // nu = 0: (U_mu(x-1) U_0(x+mu-1))^dag U_0(x-1)
+        const M3<T> X = su3_mul(lds_link(mu, lxm), ldu(U, V, 0, (unsigned)(s + fw[mu] + dlt(g, x, 0, -1))));
+        acc_mul_dn(sig, X, full_of(lds_link(0, lxm)));
+    }
+    own = lds_link(mu, lane);  // read back rather than held through the staples
+    ta_coeffs(mul(full_of(own), sig), c);
+    // pin c here: otherwise the staple arithmetic is sunk past the pairs to its use
+    asm volatile("" : "+v"(c[0]), "+v"(c[1]), "+v"(c[2]), "+v"(c[3]), "+v"(c[4]), "+v"(c[5]), "+v"(c[6]),
+                 "+v"(c[7]));
+    const int np = mu == 0 || mu == 3 ? 1 : 2;
+#pragma unroll 1
+    for (int q = 0; q < np; q++) {
+        // (a, b): w0 (0,3); w1 (1,2), (0,1); w2 (2,3), (0,2); w3 (1,3)
+        const int a = q == 1 ? 0 : (mu == 2 ? 2 : (mu == 3 ? 1 : mu));
+        const int b = q == 1 ? mu : (mu == 0 ? 3 : (mu == 1 ? 2 : 3));
+        // X_ab = U_b(x+a) U_a(x+b)^dag
+        const R6<T> ub = a == 0 ? lds_link(b, lxp) : ldr(U, V, b, (unsigned)(s + sel4(a, fw[0], fw[1], fw[2], fw[3])));
+        const M3<T> X = su3_mul_nd(ub, ldu(U, V, a, (unsigned)(s + sel4(b, fw[0], fw[1], fw[2], fw[3]))));
+        T q8[8];
+        ta_coeffs(su3_mul_nd(rows_of(su3_mul(lds_link(a, lane), X)), full_of(lds_link(b, lane))), q8);
+        const int p = pair_index(a, b);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            T2<T> v;
+            v.x = q8[2 * e];
+            v.y = q8[2 * e + 1];
+            tp[p][e][lane] = v;
+        }
+    }
+    __syncthreads();  // every pair's coefficients are in LDS
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int nu = j + (j >= mu ? 1 : 0);
+        const int p = pair_index(mu < nu ? mu : nu, mu < nu ? nu : mu);
+        const T sg = mu < nu ? T(1) : T(-1);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const T2<T> v = tp[p][e][lane];
+            c[2 * e] = fma(sg, v.x, c[2 * e]);
+            c[2 * e + 1] = fma(sg, v.y, c[2 * e + 1]);
+        }
+    }
+}
+
 // staple_sigma_own or, when SHARED, staple_sigma_shared (xs: the kernel's LDS array)
 template <typename T, bool SHARED>
 __device__ __forceinline__ M3<T> staple_sum_own(const Geo& g, const T* __restrict__ U, unsigned V, int s,
@@ -631,6 +711,14 @@ __device__ __forceinline__ M3<T> staple_sum_own(const Geo& g, const T* __restric
         return staple_sigma_shared(g, U, V, s, x, mu, own, xs);
     else
         return staple_sigma_own(g, U, V, s, x, mu, own);
+}
+// Stencil form of the force-type kernels (k_force, k_flow_stage): per-thread staples; upper-
+// staple plaquettes shared through LDS (every lane a site: V % 64 == 0); and the same with the
+// x-neighbour links from LDS (workgroups of whole x-rows: NX divides 64).
+enum { ST_OWN = 0, ST_SHARED = 1, ST_XROW = 2 };
+inline int stencil_form(const Geo& g) {
+    if (g.V % 64 != 0) return ST_OWN;
+    return 64 % g.n[0] == 0 ? ST_XROW : ST_SHARED;
 }
 // Site of a stencil thread.  The SHARED kernels have a workgroup barrier, so no lane may
 // leave early: they run only for V % 64 == 0, where every lane is a site; the clamp keeps

@@ -116,7 +116,9 @@ def test_dsdu_parity(Lat, oracle, dims):
     assert maxdiff(L.dsdu(COEFF), oracle.dsdu(dims, U, COEFF)) < 1e-13 * COEFF * 6
 
 
-@pytest.mark.parametrize("dims", [D4, (2, 4, 2, 6), (8, 8, 8, 8), (16, 8, 4, 8)])
+# stencil forms: per-thread staples (2,4,2,6: V not a multiple of 64), shared plaquettes
+# (12,4,4,4: NX does not divide 64), x-rows from LDS (the rest; 64,2,2,2: one row per workgroup)
+@pytest.mark.parametrize("dims", [D4, (2, 4, 2, 6), (8, 8, 8, 8), (16, 8, 4, 8), (12, 4, 4, 4), (64, 2, 2, 2)])
 def test_force_parity(Lat, oracle, dims):
     L = Lat(dims)
     U = alg_links(oracle, dims, 0.3, 12)
