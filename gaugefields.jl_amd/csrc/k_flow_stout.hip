@@ -195,49 +195,127 @@ __device__ __forceinline__ void acc_hd(M3<T>& c, const Herm<T>& h, const M3<T>& 
         }
 }
 
-// At most 168 VGPRs (FP64: 166, no spills), so that three waves fit per SIMD.
 template <typename T>
+__device__ __forceinline__ R6<T> lds_r6(XShare<T> ul, int d, int l) {
+    R6<T> r;
+#pragma unroll
+    for (int e = 0; e < 6; e++) r.v[e] = ul[d][e][l];
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ void lds_c8(PairShare<T> hl, int d, int l, T c[8]) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const T2<T> v = hl[d][e][l];
+        c[2 * e] = v.x;
+        c[2 * e + 1] = v.y;
+    }
+}
+// Plane b of K6b added into acc.  SH: U_b(y) and Lam_b(y) from the workgroup's LDS copies
+// (ul, hl) instead of memory; A0 (a = 0) and B0 (b = 0): the x-neighbour links and Lam at
+// y + 1 (lane lxp) and y - 1 (lane lxm) from LDS as well (workgroups of whole x-rows).
+template <typename T, bool SH, bool A0, bool B0>
+__device__ __forceinline__ void back_plane(const Geo& g, const T* __restrict__ U, const T* __restrict__ Lc, unsigned V,
+                                           int y, const int x[4], int a, int b, int ypa, M3<T>& acc, XShare<T> ul,
+                                           PairShare<T> hl, int lane, int lxm, int lxp) {
+    const int ypb0 = y + dlt(g, x, b, 1);
+    const int dmb = dlt(g, x, b, -1);
+    const int ymb0 = y + dmb, ypamb0 = ypa + dmb;
+    // each group loads its links through its own opaque index copies, so the compiler
+    // issues them group by group instead of hoisting all of them
+    {  // upper: i X H_a(y+b) U_b(y)^dag + i [S H_b(y) - H_b(y+a) S],  S = X U_b(y)^dag
+        int y0 = y, y1 = ypa, ypb = ypb0;
+        asm volatile("" : "+v"(y0), "+v"(y1), "+v"(ypb));
+        const R6<T> uby = A0 ? lds_r6<T>(ul, b, lxp) : ldr(U, V, b, y1);                // U_b(y+a)
+        const M3<T> X = su3_mul_nd(uby, full_of(B0 ? lds_r6<T>(ul, a, lxp) : ldr(U, V, a, ypb)));  // U_a(y+b)
+        T c[8];
+        if (B0) lds_c8<T>(hl, a, lxp, c);  // Lam_a(y+b)
+        else ldp(Lc, V, a, ypb, c);
+        const M3<T> XH = times_i(mul_mh(X, herm_of_coeffs(c)));
+        const M3<T> ub = full_of(SH ? lds_r6<T>(ul, b, lane) : ldr(U, V, b, y0));        // U_b(y)
+        acc_mul_nd(acc, XH, ub);
+        const M3<T> S = su3_mul_nd(rows_of(X), ub);
+        if (SH) lds_c8<T>(hl, b, lane, c);  // Lam_b(y)
+        else ldp(Lc, V, b, y0, c);
+        acc_i_mh<T, 1>(acc, S, herm_of_coeffs(c));
+        if (A0) lds_c8<T>(hl, b, lxp, c);  // Lam_b(y+a)
+        else ldp(Lc, V, b, y1, c);
+        acc_i_hm<T, -1>(acc, herm_of_coeffs(c), S);
+    }
+    {  // lower: U_b(y+a-b)^dag i [U_a(y-b)^dag H_d + H_b(y+a-b) U_a(y-b)^dag] U_b(y-b),
+       // H_d = H_a(y-b) - H_b(y-b)
+        int ymb = ymb0, ypamb = ypamb0;
+        asm volatile("" : "+v"(ymb), "+v"(ypamb));
+        const M3<T> ua = full_of(B0 ? lds_r6<T>(ul, a, lxm) : ldr(U, V, a, ymb));  // U_a(y-b)
+        T ca[8], cb[8];
+        if (B0) {
+            lds_c8<T>(hl, a, lxm, ca);
+            lds_c8<T>(hl, 0, lxm, cb);
+        } else {
+            ldp(Lc, V, a, ymb, ca);
+            ldp(Lc, V, b, ymb, cb);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) ca[k] -= cb[k];
+        M3<T> A = mul_dh(ua, herm_of_coeffs(ca));
+        acc_hd(A, herm_at(Lc, V, b, ypamb), ua);
+        const M3<T> ubm = full_of(B0 ? lds_r6<T>(ul, 0, lxm) : ldr(U, V, b, ymb));  // U_b(y-b)
+        acc_mul_dn(acc, ldu(U, V, b, ypamb), times_i(mul(A, ubm)));
+    }
+}
+
+// ST (stencil_form): ST_OWN every link and Lam from memory; ST_SHARED U(y) and Lam(y) of the
+// workgroup's sites through LDS (per wave 6 link and 3 Lam loads fewer); ST_XROW also the
+// x-neighbours (6 + 3 more).  At most 168 VGPRs (FP64), so that three waves fit per SIMD.
+template <typename T, int ST>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_stout_back_stencil(
     Geo g, const T* __restrict__ U, const T* __restrict__ Lc, T* __restrict__ Din, double rho) {
+    __shared__ T2<T> ul[ST ? 4 : 1][6][64];
+    __shared__ T2<T> hl[ST ? 4 : 1][4][64];
     const SiteMu sm = site_mu_index(g);
-    const int y = sm.s, a = sm.mu;
-    if (y >= g.V) return;
+    const int a = sm.mu;
+    int y;
+    if (!stencil_site<(ST > 0)>(g, sm, y)) return;
     int x[4];
     decode(g, y, x);
     const unsigned V = g.V;
+    const int lane = (int)(threadIdx.x & 63u);
+    if constexpr (ST > 0) {
+        const R6<T> u = ldr(U, V, a, (unsigned)y);
+        T c[8];
+        ldp(Lc, V, a, (unsigned)y, c);
+#pragma unroll
+        for (int e = 0; e < 6; e++) ul[a][e][lane] = u.v[e];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            T2<T> v;
+            v.x = c[2 * e];
+            v.y = c[2 * e + 1];
+            hl[a][e][lane] = v;
+        }
+        __syncthreads();
+    }
+    const int nxm1 = g.n[0] - 1;
+    const int lxm = xlane(lane, nxm1, -1), lxp = xlane(lane, nxm1, 1);
     M3<T> acc = zero3<T>();
     const int ypa = y + dlt(g, x, a, 1);
+    if constexpr (ST == ST_XROW) {
+        if (a == 0) {
 #pragma unroll 1
-    for (int b = 0; b < 4; b++) {
-        if (b == a) continue;
-        const int ypb0 = y + dlt(g, x, b, 1);
-        const int dmb = dlt(g, x, b, -1);
-        const int ymb0 = y + dmb, ypamb0 = ypa + dmb;
-        // each group loads its links through its own opaque index copies, so the compiler
-        // issues them group by group instead of hoisting all of them
-        {  // upper: i X H_a(y+b) U_b(y)^dag + i [S H_b(y) - H_b(y+a) S],  S = X U_b(y)^dag
-            int y0 = y, y1 = ypa, ypb = ypb0;
-            asm volatile("" : "+v"(y0), "+v"(y1), "+v"(ypb));
-            const M3<T> X = su3_mul_nd(ldr(U, V, b, y1), ldu(U, V, a, ypb));
-            const M3<T> ub = ldu(U, V, b, y0);
-            acc_mul_nd(acc, times_i(mul_mh(X, herm_at(Lc, V, a, ypb))), ub);
-            const M3<T> S = su3_mul_nd(rows_of(X), ub);
-            acc_i_mh<T, 1>(acc, S, herm_at(Lc, V, b, y0));
-            acc_i_hm<T, -1>(acc, herm_at(Lc, V, b, y1), S);
+            for (int b = 1; b < 4; b++) back_plane<T, true, true, false>(g, U, Lc, V, y, x, a, b, ypa, acc, ul, hl, lane, lxm, lxp);
+        } else {
+            back_plane<T, true, false, true>(g, U, Lc, V, y, x, a, 0, ypa, acc, ul, hl, lane, lxm, lxp);
+#pragma unroll 1
+            for (int b = 1; b < 4; b++) {
+                if (b == a) continue;
+                back_plane<T, true, false, false>(g, U, Lc, V, y, x, a, b, ypa, acc, ul, hl, lane, lxm, lxp);
+            }
         }
-        {  // lower: U_b(y+a-b)^dag i [U_a(y-b)^dag H_d + H_b(y+a-b) U_a(y-b)^dag] U_b(y-b),
-           // H_d = H_a(y-b) - H_b(y-b)
-            int ymb = ymb0, ypamb = ypamb0;
-            asm volatile("" : "+v"(ymb), "+v"(ypamb));
-            const M3<T> ua = ldu(U, V, a, ymb);
-            T ca[8], cb[8];
-            ldp(Lc, V, a, ymb, ca);
-            ldp(Lc, V, b, ymb, cb);
-#pragma unroll
-            for (int k = 0; k < 8; k++) ca[k] -= cb[k];
-            M3<T> A = mul_dh(ua, herm_of_coeffs(ca));
-            acc_hd(A, herm_at(Lc, V, b, ypamb), ua);
-            acc_mul_dn(acc, ldu(U, V, b, ypamb), times_i(mul(A, ldu(U, V, b, ymb))));
+    } else {
+#pragma unroll 1
+        for (int b = 0; b < 4; b++) {
+            if (b == a) continue;
+            back_plane<T, ST == ST_SHARED, false, false>(g, U, Lc, V, y, x, a, b, ypa, acc, ul, hl, lane, lxm, lxp);
         }
     }
     M3<T> dn = ldm(Din, V, a, y);
@@ -287,7 +365,13 @@ hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const
         k_stout_back_local<T, false, false><<<nb, kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, Lc, rho);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    k_stout_back_stencil<T><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, Uin, Lc, Din, rho);
+    const int form = stencil_form(g);
+    if (form == ST_XROW)
+        k_stout_back_stencil<T, ST_XROW><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho);
+    else if (form == ST_SHARED)
+        k_stout_back_stencil<T, ST_SHARED><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho);
+    else
+        k_stout_back_stencil<T, ST_OWN><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho);
     return hipGetLastError();
 }
 
