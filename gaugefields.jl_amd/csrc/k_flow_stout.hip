@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 template <typename T, bool STORE, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_stout_forward(Geo g, const T* __restrict__ Uin, T* __restrict__ Uout,
                                                           T* __restrict__ Cst, double rho) {
-    __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
+    __shared__ T2<T> xs[SHARED ? 6 : 1][6][64];
     const SiteMu sm = site_mu_index(g);
     const int mu = sm.mu;
     int s;
@@ -90,7 +90,7 @@ template <typename T, bool STORED, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __restrict__ Uin,
                                                              const T* __restrict__ Cst, const T* __restrict__ Dout,
                                                              T* __restrict__ Din, T* __restrict__ Lc, double rho) {
-    __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
+    __shared__ T2<T> xs[SHARED ? 6 : 1][6][64];
     const SiteMu sm = site_mu_index(g);
     const int mu = sm.mu;
     int s;

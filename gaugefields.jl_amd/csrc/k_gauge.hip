@@ -69,7 +69,7 @@ __global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict_
 // SHARED: upper-staple products shared between directions through LDS (V % 64 == 0)
 template <typename T, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U, double coeff, T* __restrict__ D) {
-    __shared__ T2<T> xs[SHARED ? 7 : 1][6][64];
+    __shared__ T2<T> xs[SHARED ? 6 : 1][6][64];
     const SiteMu sm = site_mu_index(g);
     int s;
     if (!stencil_site<SHARED>(g, sm, s)) return;

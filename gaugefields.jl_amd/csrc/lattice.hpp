@@ -426,11 +426,10 @@ __device__ __forceinline__ M3<T> staple_sigma_own(const Geo& g, const T* __restr
 // unordered pair is formed once, its rows 0 and 1 are written to LDS, and after one workgroup
 // barrier every wave reads the three it needs: per site 12 of 76 link loads and 6 of 24 staple
 // products fewer than staple_sigma_own.  Pair (a, b), a < b, is formed by wave
-// w0: (0,1), (0,3)  w1: (1,2), (1,3)  w2: (0,2)  w3: (2,3); waves 2 and 3 form a second product
-// as well (pair (2,3) again, into the unused slot 6) so that all four waves run the same
-// branch-free pipelined load sequence.  Requires every lane of the workgroup to reach the
-// barrier: the kernels use it only when V is a multiple of 64 (no idle lanes).
-// LDS: xs[7][6][64] complex (43 KB FP64, 22 KB FP32).
+// w0: (0,1), (0,3)  w1: (1,2), (1,3)  w2: (0,2)  w3: (2,3), in a loop of its own ahead of the
+// pipelined lower staples.  Requires every lane of the workgroup to reach the barrier: the
+// kernels use it only when V is a multiple of 64 (no idle lanes).
+// LDS: xs[6][6][64] complex (37 KB FP64, 18 KB FP32).
 __device__ __forceinline__ int pair_index(int a, int b) {  // a < b
     return a == 0 ? b - 1 : (a == 1 ? b + 1 : 5);
 }
@@ -440,49 +439,18 @@ template <typename T>
 __device__ __forceinline__ M3<T> staple_sigma_shared(const Geo& g, const T* __restrict__ U, unsigned V, int s,
                                                      const int x[4], int mu, R6<T>& own, XShare<T> xs) {
     const int lane = (int)(threadIdx.x & 63u);
-    int fw[4], bw[4];
+    int fw[4];
 #pragma unroll
-    for (int d = 0; d < 4; d++) {
-        fw[d] = dlt(g, x, d, 1);
-        bw[d] = dlt(g, x, d, -1);
-    }
-    const int fmu = sel4(mu, fw[0], fw[1], fw[2], fw[3]);
-    // pairs computed by this wave: w0 (0,1),(0,3); w1 (1,2),(1,3); w2 (0,2); w3 (2,3)
-    const int pa1 = mu == 2 ? 0 : (mu == 3 ? 2 : mu);
-    const int pb1 = mu == 0 ? 1 : (mu == 1 ? 2 : (mu == 2 ? 2 : 3));
-    const int pa2 = mu < 2 ? mu : 2;
-    const int pb2 = 3;
-    const int p1 = pair_index(pa1, pb1), p2 = mu < 2 ? pair_index(pa2, pb2) : 6;
-    // slots: 0..3 pair links, 4..12 lower staples (3 per nu), 13..15 U_nu(x), 16 U_mu(x)
-    auto issue = [&](int k, R6<T>& dst) {
-        if (k < 4) {
-            const int a = k < 2 ? pa1 : pa2, b = k < 2 ? pb1 : pb2;
-            const int fa = sel4(a, fw[0], fw[1], fw[2], fw[3]), fb = sel4(b, fw[0], fw[1], fw[2], fw[3]);
-            if ((k & 1) == 0)
-                dst = ldr(U, V, b, (unsigned)(s + fa));  // U_b(x+a)
-            else
-                dst = ldr(U, V, a, (unsigned)(s + fb));  // U_a(x+b)
-            return;
-        }
-        if (k == 16) {
-            dst = ldr(U, V, mu, (unsigned)s);
-            return;
-        }
-        if (k >= 13) {
-            const int j = k - 13, nu = j + (j >= mu ? 1 : 0);
-            dst = ldr(U, V, nu, (unsigned)s);
-            return;
-        }
-        const int j = (k - 4) / 3, r = (k - 4) % 3, nu = j + (j >= mu ? 1 : 0);
-        const int bnu = sel4(nu, bw[0], bw[1], bw[2], bw[3]);
-        if (r == 0)
-            dst = ldr(U, V, mu, (unsigned)(s + bnu));        // U_mu(x-nu)
-        else if (r == 1)
-            dst = ldr(U, V, nu, (unsigned)(s + fmu + bnu));  // U_nu(x+mu-nu)
-        else
-            dst = ldr(U, V, nu, (unsigned)(s + bnu));        // U_nu(x-nu)
-    };
-    auto put = [&](int p, const M3<T>& X) {
+    for (int d = 0; d < 4; d++) fw[d] = dlt(g, x, d, 1);
+    const int np = mu < 2 ? 2 : 1;
+#pragma unroll 1
+    for (int q = 0; q < np; q++) {
+        // w0 (0,1),(0,3); w1 (1,2),(1,3); w2 (0,2); w3 (2,3)
+        const int a = q == 1 ? mu : (mu == 2 ? 0 : (mu == 3 ? 2 : mu));
+        const int b = q == 1 ? 3 : (mu == 0 ? 1 : (mu == 1 ? 2 : (mu == 2 ? 2 : 3)));
+        const int fa = sel4(a, fw[0], fw[1], fw[2], fw[3]), fb = sel4(b, fw[0], fw[1], fw[2], fw[3]);
+        const M3<T> X = su3_mul_nd(ldr(U, V, b, (unsigned)(s + fa)), ldu(U, V, a, (unsigned)(s + fb)));
+        const int p = pair_index(a, b);
 #pragma unroll
         for (int e = 0; e < 6; e++) {
             T2<T> v;
@@ -490,58 +458,45 @@ __device__ __forceinline__ M3<T> staple_sigma_shared(const Geo& g, const T* __re
             v.y = X.m[e].i;
             xs[p][e][lane] = v;
         }
+    }
+    M3<T> sig = staple_sigma_pipe<T, false, true, 0>(g, U, V, s, x, mu, nullptr);
+    __syncthreads();  // every pair's X is in LDS
+    // upper staples X_mu,nu U_nu(x)^dag, then U_mu(x); the links loaded one ahead
+    auto issue = [&](int k) -> R6<T> {
+        if (k == 3) return ldr(U, V, mu, (unsigned)s);
+        const int nu = k + (k >= mu ? 1 : 0);
+        return ldr(U, V, nu, (unsigned)s);
     };
-    M3<T> sig = zero3<T>();
-    M3<T> X;
-    R6<T> A, nxt;
-    issue(0, nxt);
+    R6<T> nxt = issue(0);
 #pragma unroll
-    for (int k = 0; k < 17; k++) {
+    for (int k = 0; k < 4; k++) {
         const R6<T> cur = nxt;
-        if (k + 1 < 17) issue(k + 1, nxt);
+        if (k + 1 < 4) nxt = issue(k + 1);
         __builtin_amdgcn_sched_barrier(0);
-        if (k == 16) {
+        if (k == 3) {
             own = cur;
             break;
         }
-        if (k < 4) {
-            if ((k & 1) == 0)
-                A = cur;
-            else
-                put(k < 2 ? p1 : p2, su3_mul_nd(A, full_of(cur)));
-            continue;
-        }
-        if (k < 13) {
-            const int r = (k - 4) % 3;
-            if (r == 0) A = cur;
-            else if (r == 1) X = su3_mul(A, full_of(cur));
-            else acc_mul_dn(sig, X, full_of(cur));
-            if (k == 12) __syncthreads();
-            continue;
-        }
-        {
-            const int j = k - 13, nu = j + (j >= mu ? 1 : 0);
-            const int a = mu < nu ? mu : nu, b = mu < nu ? nu : mu;
-            const int p = pair_index(a, b);
-            M3<T> M;
+        const int nu = k + (k >= mu ? 1 : 0);
+        const int p = pair_index(mu < nu ? mu : nu, mu < nu ? nu : mu);
+        M3<T> M;
 #pragma unroll
-            for (int e = 0; e < 6; e++) {
-                const T2<T> v = xs[p][e][lane];
-                M.m[e].r = v.x;
-                M.m[e].i = v.y;
+        for (int e = 0; e < 6; e++) {
+            const T2<T> v = xs[p][e][lane];
+            M.m[e].r = v.x;
+            M.m[e].i = v.y;
+        }
+        complete_row2(M);
+        const bool t = mu > nu;  // X_mu,nu = (X_nu,mu)^dag
+        M3<T> Mt;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int jj = 0; jj < 3; jj++) {
+                Mt.m[3 * i + jj].r = t ? M.m[3 * jj + i].r : M.m[3 * i + jj].r;
+                Mt.m[3 * i + jj].i = t ? -M.m[3 * jj + i].i : M.m[3 * i + jj].i;
             }
-            complete_row2(M);
-            const bool t = mu > nu;  // X_mu,nu = (X_nu,mu)^dag
-            M3<T> Mt;
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int jj = 0; jj < 3; jj++) {
-                    Mt.m[3 * i + jj].r = t ? M.m[3 * jj + i].r : M.m[3 * i + jj].r;
-                    Mt.m[3 * i + jj].i = t ? -M.m[3 * jj + i].i : M.m[3 * i + jj].i;
-                }
-            acc_mul_nd(sig, Mt, full_of(cur));
-        }
+        acc_mul_nd(sig, Mt, full_of(cur));
     }
     return sig;
 }

@@ -238,14 +238,17 @@ def test_stout_smear_and_force_parity(Lat, oracle, mode):
     assert rel(H, oracle.stout_potential(D4, U, 3, 0.1, COEFF)) < 1e-12  # momenta are zero
 
 
-def test_stout_backward_level_parity(Lat, oracle):
-    L = Lat()
-    U = gen.hot_start_links(D4, 42)
-    D = gen.hot_start_links(D4, 43)  # any link-shaped derivative
+# stencil forms of the backward stencil: x-rows from LDS (4^4), the site's own U and Lam from LDS
+# (12,4,4,4: NX does not divide 64), every load from memory (2,4,2,6: V not a multiple of 64)
+@pytest.mark.parametrize("dims", [D4, (12, 4, 4, 4), (2, 4, 2, 6)])
+def test_stout_backward_level_parity(Lat, oracle, dims):
+    L = Lat(dims)
+    U = gen.hot_start_links(dims, 42)
+    D = gen.hot_start_links(dims, 43)  # any link-shaped derivative
     L.upload_links(U)
     L.stout_config(1, 0.12, "store")
     L.stout_smear(download=False)
-    assert maxdiff(L.stout_backward(D), oracle.stout_backward(D4, U, 0.12, D)) < 1e-13
+    assert maxdiff(L.stout_backward(D), oracle.stout_backward(dims, U, 0.12, D)) < 1e-13
 
 
 def test_stout_md_parity(Lat, oracle):
