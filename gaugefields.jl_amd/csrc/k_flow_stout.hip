@@ -86,17 +86,51 @@ __global__ __launch_bounds__(kBlock) void k_stout_forward(Geo g, const T* __rest
 // uses G_mu = U_mu^dag Lam_mu (= dS/dC_mu) without storing it.
 // STORED reads C_mu from the forward pass, otherwise it is recomputed from the staples.
 // ---------------------------------------------------------------------------
+// A 3x3 field element kept in the wave's own LDS slice between two uses (each lane reads back what
+// it wrote: no barrier), instead of being held in registers or read from memory twice.
+template <typename T>
+using MStage = T2<T> (*)[9][64];
+template <typename T>
+__device__ __forceinline__ M3<T> stage_m3(MStage<T> st, int mu, int lane, const M3<T>& m) {
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        T2<T> v;
+        v.x = m.m[e].r;
+        v.y = m.m[e].i;
+        st[mu][e][lane] = v;
+    }
+    return m;
+}
+template <typename T>
+__device__ __forceinline__ M3<T> staged_m3(MStage<T> st, int mu, int lane) {
+    // an opaque copy of the index, so that the compiler reads LDS rather than forwarding the
+    // stored value (which would keep it live in registers)
+    asm volatile("" : "+v"(lane));
+    M3<T> m;
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        const T2<T> v = st[mu][e][lane];
+        m.m[e].r = v.x;
+        m.m[e].i = v.y;
+    }
+    return m;
+}
 template <typename T, bool STORED, bool SHARED>
 __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __restrict__ Uin,
                                                              const T* __restrict__ Cst, const T* __restrict__ Dout,
                                                              T* __restrict__ Din, T* __restrict__ Lc, double rho) {
     __shared__ T2<T> xs[SHARED ? 6 : 1][6][64];
+    // Dout_mu and a stored C_mu are used before and after exp_dexp: kept in LDS in between (the
+    // second read from memory missed L2 for a third of the kernel's HBM reads)
+    __shared__ T2<T> dst[4][9][64];
+    __shared__ T2<T> cst[STORED ? 4 : 1][9][64];
     const SiteMu sm = site_mu_index(g);
     const int mu = sm.mu;
+    const int lane = (int)(threadIdx.x & 63u);
     int s;
     if (!stencil_site<SHARED>(g, sm, s)) return;
     const unsigned V = g.V;
-    M3<T> C;  // recomputed C; a stored C is loaded where it is used instead of kept live
+    M3<T> C;  // recomputed C; a stored C is staged in LDS instead of kept live
     R6<T> own;
     if (STORED) {
         own = ldr(Uin, V, mu, s);
@@ -108,7 +142,8 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
     M3<T> E, L;
     {
         const M3<T> u = full_of(own);
-        exp_dexp(ta_mat(mul_nd(STORED ? ldm(Cst, V, mu, s) : C, u)), mul(u, ldm(Dout, V, mu, s)), E, L);
+        const M3<T> Cq = STORED ? stage_m3<T>(cst, mu, lane, ldm(Cst, V, mu, s)) : C;
+        exp_dexp(ta_mat(mul_nd(Cq, u)), mul(u, stage_m3<T>(dst, mu, lane, ldm(Dout, V, mu, s))), E, L);
     }
     const M3<T> Lam = ta_mat(L);
     {
@@ -116,11 +151,8 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
         ta_coeffs(Lam, c);
         stp<T, GF_ST_NT>(Lc, V, mu, s, c);
     }
-    // re-read Dout_mu (L1/L2 hit) instead of keeping it live across exp_dexp
-    int s1 = s;
-    asm volatile("" : "+v"(s1));
-    M3<T> D = mul(ldm(Dout, V, mu, s1), E);
-    acc_dag(D, mul(Lam, STORED ? ldm(Cst, V, mu, s1) : C));
+    M3<T> D = mul(staged_m3<T>(dst, mu, lane), E);
+    acc_dag(D, mul(Lam, STORED ? staged_m3<T>(cst, mu, lane) : C));
     stm<T, GF_ST_NT>(Din, V, mu, s, D);
 }
 
