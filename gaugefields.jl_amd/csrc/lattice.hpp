@@ -9,11 +9,17 @@
 //                    staple sums); plane (mu*9 + 3*i + j) holds element (i, j)
 //   momentum field : 4 directions x 4 pair planes x V sites of T2; plane (mu*4 + j) holds the
 //                    coefficients (2j, 2j+1) of i lambda_a/2
+// FP32 fields pair the planes: "quad plane" q of a direction holds elements 2q and 2q+1 of
+// every site as 16 contiguous bytes (a link 3 quad planes, a momentum 2, a general matrix 4 and
+// a pair plane for element 8), so that one 16-byte load per lane moves two elements in either
+// precision; the bytes per direction are the same as with pair planes.
 // Consecutive sites x, x+1, ... are consecutive in every plane, so one wavefront of 64
-// sites reads each plane with one fully coalesced 1 KiB (FP64) / 512 B (FP32) access.
+// sites reads each plane with one fully coalesced 1 KiB access.
 // Site order: s = x + NX*(y + NY*(z + NZ*t)), periodic in all four directions.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "su3.hpp"
 
@@ -179,8 +185,34 @@ __device__ __forceinline__ void buf_st(float2 v, __amdgpu_buffer_rsrc_t r, unsig
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(HIP_vector_type<unsigned, 2>::Native_vec_, v), r,
                                           (int)voff, (int)soff, AUX);
 }
+// FP32 fields keep complex elements (2q, 2q+1) of a site together in one 16-byte slot of "quad
+// plane" q (a last odd element in a pair plane after them): one 64-lane load moves two elements,
+// half the vector-memory instructions of pair planes, and the texture addresser, not HBM, bounds
+// the stencil kernels.  FP64 elements are 16 bytes already and stay in pair planes.
+__device__ __forceinline__ void buf_ld_quad(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float2& a,
+                                            float2& b) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 v = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+    a.x = v[0];
+    a.y = v[1];
+    b.x = v[2];
+    b.y = v[3];
+}
+template <int AUX = GF_ST_DEFAULT>
+__device__ __forceinline__ void buf_st_quad(float2 a, float2 b, __amdgpu_buffer_rsrc_t r, unsigned voff,
+                                            unsigned soff) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 v;
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(HIP_vector_type<unsigned, 4>::Native_vec_, v), r,
+                                           (int)voff, (int)soff, AUX);
+}
 
-// SU(3) link field: planes (mu*6 + k), k = 3*i + j < 6, hold rows 0 and 1 of U_mu.
+// SU(3) link field: planes (mu*6 + k), k = 3*i + j < 6, hold rows 0 and 1 of U_mu (FP32: quad
+// planes (mu*3 + q) hold elements 2q, 2q+1).
 // R6 is a link as loaded (its two stored rows); full_of() rebuilds the third row.
 template <typename T>
 struct R6 {
@@ -189,10 +221,16 @@ struct R6 {
 template <typename T>
 __device__ __forceinline__ R6<T> ldr(const T* __restrict__ p, unsigned V, int mu, unsigned s) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 6u * mu, 6u);
-    const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
     R6<T> o;
+    if constexpr (std::is_same<T, float>::value) {
+        const unsigned off = s * 16u, ps = V * 16u;
 #pragma unroll
-    for (int k = 0; k < 6; k++) o.v[k] = buf_ld(r, off, k * ps, (const T*)nullptr);
+        for (int q = 0; q < 3; q++) buf_ld_quad(r, off, q * ps, o.v[2 * q], o.v[2 * q + 1]);
+    } else {
+        const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
+#pragma unroll
+        for (int k = 0; k < 6; k++) o.v[k] = buf_ld(r, off, k * ps, (const T*)nullptr);
+    }
     return o;
 }
 // row2_k = conj(row0_{k+1} row1_{k+2} - row0_{k+2} row1_{k+1}): exact for SU(3) (unitary, det 1)
@@ -273,30 +311,48 @@ __device__ __forceinline__ M3<T> su3_mul(const R6<T>& a, const M3<T>& b) {  // a
     return c;
 }
 
+template <typename T>
+__device__ __forceinline__ T2<T> t2_of(const cx<T>& c) {
+    T2<T> v;
+    v.x = c.r;
+    v.y = c.i;
+    return v;
+}
 template <typename T, int AUX = GF_ST_DEFAULT>
 __device__ __forceinline__ void stu(T* __restrict__ p, unsigned V, int mu, unsigned s, const M3<T>& m) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 6u * mu, 6u);
-    const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
+    if constexpr (std::is_same<T, float>::value) {
+        const unsigned off = s * 16u, ps = V * 16u;
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        T2<T> v;
-        v.x = m.m[k].r;
-        v.y = m.m[k].i;
-        buf_st<AUX>(v, r, off, k * ps);
+        for (int q = 0; q < 3; q++) buf_st_quad<AUX>(t2_of(m.m[2 * q]), t2_of(m.m[2 * q + 1]), r, off, q * ps);
+    } else {
+        const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
+#pragma unroll
+        for (int k = 0; k < 6; k++) buf_st<AUX>(t2_of(m.m[k]), r, off, k * ps);
     }
 }
 
-// general matrix field: plane (mu*9 + k) holds complex element k = 3*i + j for all sites
+// general matrix field: plane (mu*9 + k) holds complex element k = 3*i + j for all sites (FP32:
+// quad planes 0..3 of the direction hold elements 2q, 2q+1, a pair plane after them element 8)
 template <typename T>
 __device__ __forceinline__ M3<T> ldm(const T* __restrict__ p, unsigned V, int mu, unsigned s) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 9u * mu, 9u);
-    const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
     M3<T> m;
+    T2<T> v[9];
+    if constexpr (std::is_same<T, float>::value) {
+        const unsigned ps = V * 16u;
+#pragma unroll
+        for (int q = 0; q < 4; q++) buf_ld_quad(r, s * 16u, q * ps, v[2 * q], v[2 * q + 1]);
+        v[8] = buf_ld(r, s * 8u, 4u * ps, (const T*)nullptr);
+    } else {
+        const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[k] = buf_ld(r, off, k * ps, (const T*)nullptr);
+    }
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        const T2<T> v = buf_ld(r, off, k * ps, (const T*)nullptr);
-        m.m[k].r = v.x;
-        m.m[k].i = v.y;
+        m.m[k].r = v[k].x;
+        m.m[k].i = v[k].y;
     }
     return m;
 }
@@ -304,39 +360,57 @@ __device__ __forceinline__ M3<T> ldm(const T* __restrict__ p, unsigned V, int mu
 template <typename T, int AUX = GF_ST_DEFAULT>
 __device__ __forceinline__ void stm(T* __restrict__ p, unsigned V, int mu, unsigned s, const M3<T>& m) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 9u * mu, 9u);
-    const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
+    if constexpr (std::is_same<T, float>::value) {
+        const unsigned ps = V * 16u;
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        T2<T> v;
-        v.x = m.m[k].r;
-        v.y = m.m[k].i;
-        buf_st<AUX>(v, r, off, k * ps);
+        for (int q = 0; q < 4; q++) buf_st_quad<AUX>(t2_of(m.m[2 * q]), t2_of(m.m[2 * q + 1]), r, s * 16u, q * ps);
+        buf_st<AUX>(t2_of(m.m[8]), r, s * 8u, 4u * ps);
+    } else {
+        const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
+#pragma unroll
+        for (int k = 0; k < 9; k++) buf_st<AUX>(t2_of(m.m[k]), r, off, k * ps);
     }
 }
 
-// momentum field: plane (mu*4 + j) holds coefficients (2j, 2j+1) of P_mu for all sites
+// momentum field: plane (mu*4 + j) holds coefficients (2j, 2j+1) of P_mu for all sites (FP32:
+// quad planes (mu*2 + q) hold coefficients 4q .. 4q+3)
 template <typename T>
 __device__ __forceinline__ void ldp(const T* __restrict__ p, unsigned V, int mu, unsigned s, T c[8]) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 4u * mu, 4u);
-    const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
+    T2<T> v[4];
+    if constexpr (std::is_same<T, float>::value) {
+        const unsigned ps = V * 16u;
+#pragma unroll
+        for (int q = 0; q < 2; q++) buf_ld_quad(r, s * 16u, q * ps, v[2 * q], v[2 * q + 1]);
+    } else {
+        const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = buf_ld(r, off, j * ps, (const T*)nullptr);
+    }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        const T2<T> v = buf_ld(r, off, j * ps, (const T*)nullptr);
-        c[2 * j] = v.x;
-        c[2 * j + 1] = v.y;
+        c[2 * j] = v[j].x;
+        c[2 * j + 1] = v[j].y;
     }
 }
 
 template <typename T, int AUX = GF_ST_DEFAULT>
 __device__ __forceinline__ void stp(T* __restrict__ p, unsigned V, int mu, unsigned s, const T c[8]) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 4u * mu, 4u);
-    const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
+    T2<T> v[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        T2<T> v;
-        v.x = c[2 * j];
-        v.y = c[2 * j + 1];
-        buf_st<AUX>(v, r, off, j * ps);
+        v[j].x = c[2 * j];
+        v[j].y = c[2 * j + 1];
+    }
+    if constexpr (std::is_same<T, float>::value) {
+        const unsigned ps = V * 16u;
+#pragma unroll
+        for (int q = 0; q < 2; q++) buf_st_quad<AUX>(v[2 * q], v[2 * q + 1], r, s * 16u, q * ps);
+    } else {
+        const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
+#pragma unroll
+        for (int j = 0; j < 4; j++) buf_st<AUX>(v[j], r, off, j * ps);
     }
 }
 
