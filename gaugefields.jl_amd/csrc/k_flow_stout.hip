@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     const M3<T> w = full_of(own);
     if (beta != T(0)) {
         T a[8];
-        ldp(A, V, mu, s, a);
+        ldp<T, GF_LD_NT>(A, V, mu, s, a);
 #pragma unroll
         for (int k = 0; k < 8; k++) c[k] = (T)fma(alpha, (double)c[k], beta * (double)a[k]);
     } else {
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
     M3<T> C;  // recomputed C; a stored C is staged in LDS instead of kept live
     R6<T> own;
     if (STORED) {
-        own = ldr(Uin, V, mu, s);
+        own = ldr<T, GF_LD_NT>(Uin, V, mu, s);
     } else {  // the same staple evaluation (and rounding) as the forward kernel
         int x[4];
         decode(g, s, x);
@@ -142,8 +142,8 @@ __global__ __launch_bounds__(kBlock) void k_stout_back_local(Geo g, const T* __r
     M3<T> E, L;
     {
         const M3<T> u = full_of(own);
-        const M3<T> Cq = STORED ? stage_m3<T>(cst, mu, lane, ldm(Cst, V, mu, s)) : C;
-        exp_dexp(ta_mat(mul_nd(Cq, u)), mul(u, stage_m3<T>(dst, mu, lane, ldm(Dout, V, mu, s))), E, L);
+        const M3<T> Cq = STORED ? stage_m3<T>(cst, mu, lane, ldm<T, GF_LD_NT>(Cst, V, mu, s)) : C;
+        exp_dexp(ta_mat(mul_nd(Cq, u)), mul(u, stage_m3<T>(dst, mu, lane, ldm<T, GF_LD_NT>(Dout, V, mu, s))), E, L);
     }
     const M3<T> Lam = ta_mat(L);
     {
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             back_plane<T, ST == ST_SHARED, false, false>(g, U, Lc, V, y, x, a, b, ypa, acc, ul, hl, lane, lxm, lxp);
         }
     }
-    M3<T> dn = ldm(Din, V, a, y);
+    M3<T> dn = ldm<T, GF_LD_NT>(Din, V, a, y);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         dn.m[k].r = fma1(rho, acc.m[k].r, dn.m[k].r);

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         stp<T, GF_ST_NT>(FP, V, mu, s, c);
     } else {
         T p[8];
-        ldp(FP, V, mu, s, p);
+        ldp<T, GF_LD_NT>(FP, V, mu, s, p);
 #pragma unroll
         for (int a = 0; a < 8; a++) p[a] = fma1(fac, c[a], p[a]);
         stp<T, GF_ST_NT>(FP, V, mu, s, p);
@@ -148,8 +148,8 @@ __global__ __launch_bounds__(kBlock) void k_exp_update(Geo g, const T* __restric
 #pragma unroll 1
     for (int mu = 0; mu < 4; mu++) {
         T p[8];
-        ldp(P, V, mu, s, p);
-        const M3<T> un = exp_coeffs_times(p, eps, ldu(Uin, V, mu, s));
+        ldp<T, GF_LD_NT>(P, V, mu, s, p);
+        const M3<T> un = exp_coeffs_times(p, eps, ldu<T, GF_LD_NT>(Uin, V, mu, s));
         stu<T, GF_ST_NT>(Uout, V, mu, s, un);
     }
 }
@@ -165,14 +165,14 @@ __global__ __launch_bounds__(kBlock) void k_force_from_deriv(Geo g, const T* __r
     const unsigned V = g.V;
     constexpr double fac = -1.0 / 3.0;
     T c[8];
-    ta_coeffs(mul(ldu(U, V, mu, s), ldm(D, V, mu, s)), c);
+    ta_coeffs(mul(ldu<T, GF_LD_NT>(U, V, mu, s), ldm<T, GF_LD_NT>(D, V, mu, s)), c);
     if (MODE == FORCE_STORE) {
 #pragma unroll
         for (int a = 0; a < 8; a++) c[a] = scale1(fac, c[a]);
         stp<T, GF_ST_NT>(FP, V, mu, s, c);
     } else {
         T p[8];
-        ldp(FP, V, mu, s, p);
+        ldp<T, GF_LD_NT>(FP, V, mu, s, p);
         const double f = eps * fac;
 #pragma unroll
         for (int a = 0; a < 8; a++) p[a] = fma1(f, c[a], p[a]);

@@ -165,11 +165,17 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t planes_rsrc(const T* p, unsign
                                              GF_RSRC_WORD3);
 }
 
+// load cache policy: GF_LD_DEFAULT, or GF_LD_NT (non-temporal: inputs each thread reads once,
+// so that streaming them through L2 does not evict the neighbour links of the stencils)
+#define GF_LD_DEFAULT 0
+#define GF_LD_NT 2
+template <int AUX = GF_LD_DEFAULT>
 __device__ __forceinline__ double2 buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, const double*) {
-    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, AUX));
 }
+template <int AUX = GF_LD_DEFAULT>
 __device__ __forceinline__ float2 buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, const float*) {
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, AUX));
 }
 // store cache policy: GF_ST_DEFAULT, or GF_ST_NT (non-temporal: streamed results that this
 // launch does not read back, so they do not evict the neighbour links held in L2)
@@ -189,10 +195,11 @@ __device__ __forceinline__ void buf_st(float2 v, __amdgpu_buffer_rsrc_t r, unsig
 // plane" q (a last odd element in a pair plane after them): one 64-lane load moves two elements,
 // half the vector-memory instructions of pair planes, and the texture addresser, not HBM, bounds
 // the stencil kernels.  FP64 elements are 16 bytes already and stay in pair planes.
+template <int AUX = GF_LD_DEFAULT>
 __device__ __forceinline__ void buf_ld_quad(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float2& a,
                                             float2& b) {
     typedef float f4 __attribute__((ext_vector_type(4)));
-    const f4 v = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+    const f4 v = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, AUX));
     a.x = v[0];
     a.y = v[1];
     b.x = v[2];
@@ -218,18 +225,18 @@ template <typename T>
 struct R6 {
     T2<T> v[6];
 };
-template <typename T>
+template <typename T, int AUX = GF_LD_DEFAULT>
 __device__ __forceinline__ R6<T> ldr(const T* __restrict__ p, unsigned V, int mu, unsigned s) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 6u * mu, 6u);
     R6<T> o;
     if constexpr (std::is_same<T, float>::value) {
         const unsigned off = s * 16u, ps = V * 16u;
 #pragma unroll
-        for (int q = 0; q < 3; q++) buf_ld_quad(r, off, q * ps, o.v[2 * q], o.v[2 * q + 1]);
+        for (int q = 0; q < 3; q++) buf_ld_quad<AUX>(r, off, q * ps, o.v[2 * q], o.v[2 * q + 1]);
     } else {
         const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
 #pragma unroll
-        for (int k = 0; k < 6; k++) o.v[k] = buf_ld(r, off, k * ps, (const T*)nullptr);
+        for (int k = 0; k < 6; k++) o.v[k] = buf_ld<AUX>(r, off, k * ps, (const T*)nullptr);
     }
     return o;
 }
@@ -255,9 +262,9 @@ __device__ __forceinline__ M3<T> full_of(const R6<T>& r) {
     complete_row2(m);
     return m;
 }
-template <typename T>
+template <typename T, int AUX = GF_LD_DEFAULT>
 __device__ __forceinline__ M3<T> ldu(const T* __restrict__ p, unsigned V, int mu, unsigned s) {
-    return full_of(ldr(p, V, mu, s));
+    return full_of(ldr<T, AUX>(p, V, mu, s));
 }
 template <typename T>
 __device__ __forceinline__ cx<T> rc(const R6<T>& r, int k) {
@@ -334,7 +341,7 @@ __device__ __forceinline__ void stu(T* __restrict__ p, unsigned V, int mu, unsig
 
 // general matrix field: plane (mu*9 + k) holds complex element k = 3*i + j for all sites (FP32:
 // quad planes 0..3 of the direction hold elements 2q, 2q+1, a pair plane after them element 8)
-template <typename T>
+template <typename T, int AUX = GF_LD_DEFAULT>
 __device__ __forceinline__ M3<T> ldm(const T* __restrict__ p, unsigned V, int mu, unsigned s) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 9u * mu, 9u);
     M3<T> m;
@@ -342,12 +349,12 @@ __device__ __forceinline__ M3<T> ldm(const T* __restrict__ p, unsigned V, int mu
     if constexpr (std::is_same<T, float>::value) {
         const unsigned ps = V * 16u;
 #pragma unroll
-        for (int q = 0; q < 4; q++) buf_ld_quad(r, s * 16u, q * ps, v[2 * q], v[2 * q + 1]);
-        v[8] = buf_ld(r, s * 8u, 4u * ps, (const T*)nullptr);
+        for (int q = 0; q < 4; q++) buf_ld_quad<AUX>(r, s * 16u, q * ps, v[2 * q], v[2 * q + 1]);
+        v[8] = buf_ld<AUX>(r, s * 8u, 4u * ps, (const T*)nullptr);
     } else {
         const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
 #pragma unroll
-        for (int k = 0; k < 9; k++) v[k] = buf_ld(r, off, k * ps, (const T*)nullptr);
+        for (int k = 0; k < 9; k++) v[k] = buf_ld<AUX>(r, off, k * ps, (const T*)nullptr);
     }
 #pragma unroll
     for (int k = 0; k < 9; k++) {
@@ -374,18 +381,18 @@ __device__ __forceinline__ void stm(T* __restrict__ p, unsigned V, int mu, unsig
 
 // momentum field: plane (mu*4 + j) holds coefficients (2j, 2j+1) of P_mu for all sites (FP32:
 // quad planes (mu*2 + q) hold coefficients 4q .. 4q+3)
-template <typename T>
+template <typename T, int AUX = GF_LD_DEFAULT>
 __device__ __forceinline__ void ldp(const T* __restrict__ p, unsigned V, int mu, unsigned s, T c[8]) {
     const __amdgpu_buffer_rsrc_t r = planes_rsrc(p, V, 4u * mu, 4u);
     T2<T> v[4];
     if constexpr (std::is_same<T, float>::value) {
         const unsigned ps = V * 16u;
 #pragma unroll
-        for (int q = 0; q < 2; q++) buf_ld_quad(r, s * 16u, q * ps, v[2 * q], v[2 * q + 1]);
+        for (int q = 0; q < 2; q++) buf_ld_quad<AUX>(r, s * 16u, q * ps, v[2 * q], v[2 * q + 1]);
     } else {
         const unsigned off = s * (unsigned)sizeof(T2<T>), ps = V * (unsigned)sizeof(T2<T>);
 #pragma unroll
-        for (int j = 0; j < 4; j++) v[j] = buf_ld(r, off, j * ps, (const T*)nullptr);
+        for (int j = 0; j < 4; j++) v[j] = buf_ld<AUX>(r, off, j * ps, (const T*)nullptr);
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
