@@ -202,14 +202,17 @@ int stout_forward_all(gf_ctx* c) {
     });
 }
 
-// D (dS/dU of the top level) -> D holds dS/dU_bare afterwards
-int stout_backward_all(gf_ctx* c) {
+// D (dS/dU of the top level) -> D holds dS/dU_bare afterwards; or, with fmode FORCE_STORE /
+// FORCE_KICK, the last (thin-link) level forms the force from dS/dU_bare instead of storing it:
+// F = -(1/3) TA(U dS/dU_bare) into FP, or FP += eps F
+int stout_backward_all(gf_ctx* c, int fmode = -1, void* FP = nullptr, double eps = 0.0) {
     return dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
         for (int l = c->nlev - 1; l >= 0; l--) {
             HIPCHK(c, launch_stout_backward<T>(c->geo, (const T*)level_in(c, l),
                                                c->smode == GF_STOUT_STORE ? (const T*)c->Cst[l] : nullptr,
-                                               (const T*)c->D, (T*)c->D2, (T*)c->G, c->rho, c->stream));
+                                               (const T*)c->D, (T*)c->D2, (T*)c->G, c->rho, c->stream,
+                                               l == 0 ? fmode : -1, (T*)FP, eps));
             std::swap(c->D, c->D2);
         }
         return GF_OK;
@@ -224,11 +227,8 @@ int stout_force_impl(gf_ctx* c, double coeff, int mode, double eps) {
     return dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
         HIPCHK(c, launch_dsdu<T>(c->geo, (const T*)top_links(c), coeff, (T*)c->D, c->stream));
-        int r = stout_backward_all(c);
-        if (r != GF_OK) return r;
-        HIPCHK(c, launch_force_from_deriv<T>(mode, c->geo, (const T*)c->U, (const T*)c->D,
-                                              mode == FORCE_STORE ? (T*)c->F : (T*)c->P, eps, c->stream));
-        return GF_OK;
+        // the thin-link level's stencil kernel forms the force from dS/dU_bare
+        return stout_backward_all(c, mode, mode == FORCE_STORE ? (void*)c->F : (void*)c->P, eps);
     });
 }
 

@@ -39,9 +39,6 @@ hipError_t launch_force(int mode, const Geo& g, const T* U, T* FP, T* Unew, doub
                         double eps_u, hipStream_t st);
 template <typename T>
 hipError_t launch_exp_update(const Geo& g, const T* Uin, T* Uout, const T* P, double eps, hipStream_t st);
-template <typename T>
-hipError_t launch_force_from_deriv(int mode, const Geo& g, const T* U, const T* D, T* FP, double eps,
-                                   hipStream_t st);
 // reference-layout (column-major 3x3 per link) <-> device field; general = false: SU(3) link
 // field (rows 0 and 1 kept, row 2 rebuilt on the way out), true: general matrix field
 hipError_t launch_links_widen(const float* Uf, double* Ud, int V, hipStream_t st);
@@ -91,8 +88,11 @@ hipError_t launch_flow_stage(const Geo& g, const T* W, T* Wout, T* A, double alp
 template <typename T>
 hipError_t launch_stout_forward(const Geo& g, const T* Uin, T* Uout, T* Cst, double rho, hipStream_t st);
 template <typename T>
-// Lc: scratch momentum-layout field (the 8 su(3) coefficients of Lam per link)
+// Lc: scratch momentum-layout field (the 8 su(3) coefficients of Lam per link).
+// fmode < 0: Din = dS/dU_in.  fmode = FORCE_STORE / FORCE_KICK (the thin-link level of a stout
+// force): Din is not stored; FP = -(1/3) TA(U_in Din), or FP += eps of that, as
+// it follows from the stored Din (the stout force, test/HMCstout_test_nowing.jl:99-118).
 hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc,
-                                 double rho, hipStream_t st);
+                                 double rho, hipStream_t st, int fmode = -1, T* FP = nullptr, double eps = 0.0);
 
 }  // namespace gf

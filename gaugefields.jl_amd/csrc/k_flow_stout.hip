@@ -299,9 +299,13 @@ __device__ __forceinline__ void back_plane(const Geo& g, const T* __restrict__ U
 // ST (stencil_form): ST_OWN every link and Lam from memory; ST_SHARED U(y) and Lam(y) of the
 // workgroup's sites through LDS (per wave 6 link and 3 Lam loads fewer); ST_XROW also the
 // x-neighbours (6 + 3 more).  At most 168 VGPRs (FP64), so that three waves fit per SIMD.
-template <typename T, int ST>
+// OUT < 0: Din += rho (staple part).  OUT = FORCE_STORE / FORCE_KICK: the thin-link level of a stout
+// force (test/HMCstout_test_nowing.jl:99-118): Din is not stored but turned into the force
+// FP = -(1/3) TA(U_a(y) Din), or FP += eps of that, in the same kernel.
+template <typename T, int ST, int OUT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_stout_back_stencil(
-    Geo g, const T* __restrict__ U, const T* __restrict__ Lc, T* __restrict__ Din, double rho) {
+    Geo g, const T* __restrict__ U, const T* __restrict__ Lc, T* __restrict__ Din, double rho,
+    T* __restrict__ FP, double eps) {
     __shared__ T2<T> ul[ST ? 4 : 1][6][64];
     __shared__ T2<T> hl[ST ? 4 : 1][4][64];
     const SiteMu sm = site_mu_index(g);
@@ -356,7 +360,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         dn.m[k].r = fma1(rho, acc.m[k].r, dn.m[k].r);
         dn.m[k].i = fma1(rho, acc.m[k].i, dn.m[k].i);
     }
-    stm<T, GF_ST_NT>(Din, V, a, y, dn);
+    if constexpr (OUT < 0) {
+        stm<T, GF_ST_NT>(Din, V, a, y, dn);
+    } else {  // the force from the thin-link derivative
+        constexpr double fac = -1.0 / 3.0;
+        R6<T> ur;
+        if constexpr (ST > 0)
+            ur = lds_r6<T>(ul, a, lane);
+        else
+            ur = ldr(U, V, a, (unsigned)y);
+        const M3<T> ua = full_of(ur);
+        T c[8];
+        ta_coeffs(mul(ua, dn), c);
+        if constexpr (OUT == FORCE_STORE) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) c[k] = scale1(fac, c[k]);
+            stp<T, GF_ST_NT>(FP, V, a, y, c);
+        } else {
+            T p[8];
+            ldp<T, GF_LD_NT>(FP, V, a, y, p);
+            const double f = eps * fac;
+#pragma unroll
+            for (int k = 0; k < 8; k++) p[k] = fma1(f, c[k], p[k]);
+            stp<T, GF_ST_NT>(FP, V, a, y, p);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -385,9 +413,19 @@ hipError_t launch_stout_forward(const Geo& g, const T* Uin, T* Uout, T* Cst, dou
         k_stout_forward<T, false, false><<<nb, kBlock, 0, st>>>(g, Uin, Uout, nullptr, rho);
     return hipGetLastError();
 }
+template <typename T, int OUT>
+void launch_back_stencil(int form, int nb, hipStream_t st, const Geo& g, const T* Uin, const T* Lc, T* Din,
+                         double rho, T* FP, double eps) {
+    if (form == ST_XROW)
+        k_stout_back_stencil<T, ST_XROW, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps);
+    else if (form == ST_SHARED)
+        k_stout_back_stencil<T, ST_SHARED, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps);
+    else
+        k_stout_back_stencil<T, ST_OWN, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps);
+}
 template <typename T>
 hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc, double rho,
-                                 hipStream_t st) {
+                                 hipStream_t st, int fmode, T* FP, double eps) {
     const int nb = grid_sites_mu(g.V);
     if (Cst)
         k_stout_back_local<T, true, false><<<nb, kBlock, 0, st>>>(g, Uin, Cst, Dout, Din, Lc, rho);
@@ -398,12 +436,12 @@ hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int form = stencil_form(g);
-    if (form == ST_XROW)
-        k_stout_back_stencil<T, ST_XROW><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho);
-    else if (form == ST_SHARED)
-        k_stout_back_stencil<T, ST_SHARED><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho);
+    if (fmode == FORCE_STORE)
+        launch_back_stencil<T, FORCE_STORE>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps);
+    else if (fmode == FORCE_KICK)
+        launch_back_stencil<T, FORCE_KICK>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps);
     else
-        k_stout_back_stencil<T, ST_OWN><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho);
+        launch_back_stencil<T, -1>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps);
     return hipGetLastError();
 }
 
@@ -411,7 +449,7 @@ hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const
     template hipError_t launch_flow_stage<T>(const Geo&, const T*, T*, T*, double, double, hipStream_t);  \
     template hipError_t launch_stout_forward<T>(const Geo&, const T*, T*, T*, double, hipStream_t);       \
     template hipError_t launch_stout_backward<T>(const Geo&, const T*, const T*, const T*, T*, T*, double, \
-                                                 hipStream_t);
+                                                 hipStream_t, int, T*, double);
 
 GF_INST2(double)
 GF_INST2(float)

@@ -154,31 +154,6 @@ __global__ __launch_bounds__(kBlock) void k_exp_update(Geo g, const T* __restric
     }
 }
 
-// F_mu = -(1/3) TA(U_mu D_mu) from a matrix derivative D (stout force,
-// test/HMCstout_test_nowing.jl:99-118); store or kick
-template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_force_from_deriv(Geo g, const T* __restrict__ U, const T* __restrict__ D,
-                                                             T* __restrict__ FP, double eps) {
-    const SiteMu sm = site_mu_index(g);
-    const int s = sm.s, mu = sm.mu;
-    if (s >= g.V) return;
-    const unsigned V = g.V;
-    constexpr double fac = -1.0 / 3.0;
-    T c[8];
-    ta_coeffs(mul(ldu<T, GF_LD_NT>(U, V, mu, s), ldm<T, GF_LD_NT>(D, V, mu, s)), c);
-    if (MODE == FORCE_STORE) {
-#pragma unroll
-        for (int a = 0; a < 8; a++) c[a] = scale1(fac, c[a]);
-        stp<T, GF_ST_NT>(FP, V, mu, s, c);
-    } else {
-        T p[8];
-        ldp<T, GF_LD_NT>(FP, V, mu, s, p);
-        const double f = eps * fac;
-#pragma unroll
-        for (int a = 0; a < 8; a++) p[a] = fma1(f, c[a], p[a]);
-        stp<T, GF_ST_NT>(FP, V, mu, s, p);
-    }
-}
 
 // FP32-bulk link field -> FP64 link field (the two stored rows, exactly; row 2 is rebuilt on
 // every load in either precision)
@@ -347,17 +322,6 @@ hipError_t launch_exp_update(const Geo& g, const T* Uin, T* Uout, const T* P, do
     k_exp_update<T><<<grid_for(g.V), kBlock, 0, st>>>(g, Uin, Uout, P, eps);
     return hipGetLastError();
 }
-template <typename T>
-hipError_t launch_force_from_deriv(int mode, const Geo& g, const T* U, const T* D, T* FP, double eps,
-                                   hipStream_t st) {
-    if (mode == FORCE_STORE)
-        k_force_from_deriv<T, FORCE_STORE><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, D, FP, eps);
-    else if (mode == FORCE_KICK)
-        k_force_from_deriv<T, FORCE_KICK><<<grid_sites_mu(g.V), kBlock, 0, st>>>(g, U, D, FP, eps);
-    else
-        return hipErrorInvalidValue;
-    return hipGetLastError();
-}
 hipError_t launch_links_widen(const float* Uf, double* Ud, int V, hipStream_t st) {
     k_links_widen<<<grid_for(V), kBlock, 0, st>>>(Uf, Ud, V);
     return hipGetLastError();
@@ -413,8 +377,6 @@ hipError_t launch_axpby(T* y, const T* x, double a, double b, size_t n, hipStrea
     template hipError_t launch_force<T>(int, const Geo&, const T*, T*, T*, double, double, double,       \
                                         hipStream_t);                                                    \
     template hipError_t launch_exp_update<T>(const Geo&, const T*, T*, const T*, double, hipStream_t);   \
-    template hipError_t launch_force_from_deriv<T>(int, const Geo&, const T*, const T*, T*, double,      \
-                                                   hipStream_t);                                         \
     template hipError_t launch_links_from_ref<T>(const double*, T*, int, bool, hipStream_t);             \
     template hipError_t launch_links_to_ref<T>(const T*, double*, int, bool, hipStream_t);               \
     template hipError_t launch_momenta_from_ref<T>(const double*, T*, int, hipStream_t);                 \
