@@ -205,14 +205,14 @@ int stout_forward_all(gf_ctx* c) {
 // D (dS/dU of the top level) -> D holds dS/dU_bare afterwards; or, with fmode FORCE_STORE /
 // FORCE_KICK, the last (thin-link) level forms the force from dS/dU_bare instead of storing it:
 // F = -(1/3) TA(U dS/dU_bare) into FP, or FP += eps F
-int stout_backward_all(gf_ctx* c, int fmode = -1, void* FP = nullptr, double eps = 0.0) {
+int stout_backward_all(gf_ctx* c, int fmode = -1, void* FP = nullptr, double eps = 0.0, double eps_u = 0.0) {
     return dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
         for (int l = c->nlev - 1; l >= 0; l--) {
             HIPCHK(c, launch_stout_backward<T>(c->geo, (const T*)level_in(c, l),
                                                c->smode == GF_STOUT_STORE ? (const T*)c->Cst[l] : nullptr,
                                                (const T*)c->D, (T*)c->D2, (T*)c->G, c->rho, c->stream,
-                                               l == 0 ? fmode : -1, (T*)FP, eps));
+                                               l == 0 ? fmode : -1, (T*)FP, eps, (T*)c->Ualt, eps_u));
             std::swap(c->D, c->D2);
         }
         return GF_OK;
@@ -221,14 +221,15 @@ int stout_backward_all(gf_ctx* c, int fmode = -1, void* FP = nullptr, double eps
 
 const void* top_links(gf_ctx* c) { return c->nlev > 0 ? c->lev[c->nlev - 1] : c->U; }
 
-// mode FORCE_STORE -> F, FORCE_KICK -> P += eps F
-int stout_force_impl(gf_ctx* c, double coeff, int mode, double eps) {
+// mode FORCE_STORE -> F, FORCE_KICK -> P += eps F, FORCE_KICK_EXP -> that and Ualt = exp(eps_u P) U
+// (the caller swaps the link buffers)
+int stout_force_impl(gf_ctx* c, double coeff, int mode, double eps, double eps_u = 0.0) {
     TRY(stout_forward_all(c));
     return dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
         HIPCHK(c, launch_dsdu<T>(c->geo, (const T*)top_links(c), coeff, (T*)c->D, c->stream));
         // the thin-link level's stencil kernel forms the force from dS/dU_bare
-        return stout_backward_all(c, mode, mode == FORCE_STORE ? (void*)c->F : (void*)c->P, eps);
+        return stout_backward_all(c, mode, mode == FORCE_STORE ? (void*)c->F : (void*)c->P, eps, eps_u);
     });
 }
 
@@ -251,9 +252,10 @@ int expupd(gf_ctx* c, double eps) {
 
 // kick then exp update; one fused kernel for the plain action
 int kick_exp(gf_ctx* c, double coeff, double eps_p, double eps_u) {
-    if (c->nlev > 0) {
-        TRY(stout_force_impl(c, coeff, FORCE_KICK, eps_p));
-        return expupd(c, eps_u);
+    if (c->nlev > 0) {  // the thin-link backward stencil kicks and updates (into Ualt)
+        TRY(stout_force_impl(c, coeff, FORCE_KICK_EXP, eps_p, eps_u));
+        swap_links(c);
+        return GF_OK;
     }
     TRY(dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;

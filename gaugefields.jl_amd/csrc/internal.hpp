@@ -89,10 +89,11 @@ template <typename T>
 hipError_t launch_stout_forward(const Geo& g, const T* Uin, T* Uout, T* Cst, double rho, hipStream_t st);
 template <typename T>
 // Lc: scratch momentum-layout field (the 8 su(3) coefficients of Lam per link).
-// fmode < 0: Din = dS/dU_in.  fmode = FORCE_STORE / FORCE_KICK (the thin-link level of a stout
-// force): Din is not stored; FP = -(1/3) TA(U_in Din), or FP += eps of that, as
-// it follows from the stored Din (the stout force, test/HMCstout_test_nowing.jl:99-118).
+// fmode < 0: Din = dS/dU_in.  fmode = FORCE_STORE / FORCE_KICK / FORCE_KICK_EXP (the thin-link
+// level of a stout force, test/HMCstout_test_nowing.jl:99-118): Din is not stored; FP = -(1/3)
+// TA(U_in Din), or FP += eps of that, and with FORCE_KICK_EXP also Unew = exp(eps_u FP) U_in.
 hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc,
-                                 double rho, hipStream_t st, int fmode = -1, T* FP = nullptr, double eps = 0.0);
+                                 double rho, hipStream_t st, int fmode = -1, T* FP = nullptr, double eps = 0.0,
+                                 T* Unew = nullptr, double eps_u = 0.0);
 
 }  // namespace gf

@@ -299,13 +299,15 @@ __device__ __forceinline__ void back_plane(const Geo& g, const T* __restrict__ U
 // ST (stencil_form): ST_OWN every link and Lam from memory; ST_SHARED U(y) and Lam(y) of the
 // workgroup's sites through LDS (per wave 6 link and 3 Lam loads fewer); ST_XROW also the
 // x-neighbours (6 + 3 more).  At most 168 VGPRs (FP64), so that three waves fit per SIMD.
-// OUT < 0: Din += rho (staple part).  OUT = FORCE_STORE / FORCE_KICK: the thin-link level of a stout
-// force (test/HMCstout_test_nowing.jl:99-118): Din is not stored but turned into the force
-// FP = -(1/3) TA(U_a(y) Din), or FP += eps of that, in the same kernel.
+// OUT < 0: Din += rho (staple part).  OUT = FORCE_STORE / FORCE_KICK / FORCE_KICK_EXP: the thin-link
+// level of a stout force (test/HMCstout_test_nowing.jl:99-118): Din is not stored but turned into
+// the force FP = -(1/3) TA(U_a(y) Din), or FP += eps of that, in the same kernel; FORCE_KICK_EXP
+// then also writes Unew = exp(eps_u FP) U_a(y) (update_gaugefields!, molecular_dynamics.jl:513-531)
+// to a buffer other than U, which the other threads still read.
 template <typename T, int ST, int OUT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_stout_back_stencil(
     Geo g, const T* __restrict__ U, const T* __restrict__ Lc, T* __restrict__ Din, double rho,
-    T* __restrict__ FP, double eps) {
+    T* __restrict__ FP, double eps, T* __restrict__ Unew, double eps_u) {
     __shared__ T2<T> ul[ST ? 4 : 1][6][64];
     __shared__ T2<T> hl[ST ? 4 : 1][4][64];
     const SiteMu sm = site_mu_index(g);
@@ -383,6 +385,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 #pragma unroll
             for (int k = 0; k < 8; k++) p[k] = fma1(f, c[k], p[k]);
             stp<T, GF_ST_NT>(FP, V, a, y, p);
+            if constexpr (OUT == FORCE_KICK_EXP) stu<T, GF_ST_NT>(Unew, V, a, y, exp_coeffs_times(p, eps_u, ua));
         }
     }
 }
@@ -415,17 +418,17 @@ hipError_t launch_stout_forward(const Geo& g, const T* Uin, T* Uout, T* Cst, dou
 }
 template <typename T, int OUT>
 void launch_back_stencil(int form, int nb, hipStream_t st, const Geo& g, const T* Uin, const T* Lc, T* Din,
-                         double rho, T* FP, double eps) {
+                         double rho, T* FP, double eps, T* Unew, double eps_u) {
     if (form == ST_XROW)
-        k_stout_back_stencil<T, ST_XROW, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps);
+        k_stout_back_stencil<T, ST_XROW, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
     else if (form == ST_SHARED)
-        k_stout_back_stencil<T, ST_SHARED, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps);
+        k_stout_back_stencil<T, ST_SHARED, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
     else
-        k_stout_back_stencil<T, ST_OWN, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps);
+        k_stout_back_stencil<T, ST_OWN, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
 }
 template <typename T>
 hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc, double rho,
-                                 hipStream_t st, int fmode, T* FP, double eps) {
+                                 hipStream_t st, int fmode, T* FP, double eps, T* Unew, double eps_u) {
     const int nb = grid_sites_mu(g.V);
     if (Cst)
         k_stout_back_local<T, true, false><<<nb, kBlock, 0, st>>>(g, Uin, Cst, Dout, Din, Lc, rho);
@@ -437,11 +440,13 @@ hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const
     if (e != hipSuccess) return e;
     const int form = stencil_form(g);
     if (fmode == FORCE_STORE)
-        launch_back_stencil<T, FORCE_STORE>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps);
+        launch_back_stencil<T, FORCE_STORE>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
     else if (fmode == FORCE_KICK)
-        launch_back_stencil<T, FORCE_KICK>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps);
+        launch_back_stencil<T, FORCE_KICK>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
+    else if (fmode == FORCE_KICK_EXP)
+        launch_back_stencil<T, FORCE_KICK_EXP>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
     else
-        launch_back_stencil<T, -1>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps);
+        launch_back_stencil<T, -1>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
     return hipGetLastError();
 }
 
@@ -449,7 +454,7 @@ hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const
     template hipError_t launch_flow_stage<T>(const Geo&, const T*, T*, T*, double, double, hipStream_t);  \
     template hipError_t launch_stout_forward<T>(const Geo&, const T*, T*, T*, double, hipStream_t);       \
     template hipError_t launch_stout_backward<T>(const Geo&, const T*, const T*, const T*, T*, T*, double, \
-                                                 hipStream_t, int, T*, double);
+                                                 hipStream_t, int, T*, double, T*, double);
 
 GF_INST2(double)
 GF_INST2(float)
