@@ -251,15 +251,18 @@ def test_stout_backward_level_parity(Lat, oracle, dims):
     assert maxdiff(L.stout_backward(D), oracle.stout_backward(dims, U, 0.12, D)) < 1e-13
 
 
-def test_stout_md_parity(Lat, oracle):
-    L = Lat()
-    U = alg_links(oracle, D4, 0.3, 44)
-    P = gen.gaussian_momenta(D4, 45)
+# the fused 2MN sequence: the thin-link backward stencil forms the force and the link update, in
+# the x-row (4^4) and shared (12,4,4,4) stencil forms
+@pytest.mark.parametrize("dims", [D4, (12, 4, 4, 4)])
+def test_stout_md_parity(Lat, oracle, dims):
+    L = Lat(dims)
+    U = alg_links(oracle, dims, 0.3, 44)
+    P = gen.gaussian_momenta(dims, 45)
     L.upload_links(U)
     L.upload_momenta(P)
     L.stout_config(3, 0.1, "store")
     H = L.md_trajectory(COEFF, 4, 0.2, "2MN", fused=True)
-    Href = oracle.md_trajectory(D4, U, P, COEFF, 4, 0.2, "2MN", nlev=3, rho=0.1)
+    Href = oracle.md_trajectory(dims, U, P, COEFF, 4, 0.2, "2MN", nlev=3, rho=0.1)
     assert abs(H[2] - Href[2]) < 1e-12 * abs(Href[0])
     assert maxdiff(L.download_links(), U) < 1e-12
 
