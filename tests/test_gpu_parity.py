@@ -285,6 +285,28 @@ def test_fp32_bulk_against_fp64_oracle(Lat, oracle):
     assert abs(H[2] - Href[2]) < 1e-3
 
 
+# FP32 bulk (quad-plane storage) in the three stencil forms: 3-level stout force and stout force
+# + kick + link update (the fused 2MN sequence) against the FP64 oracle on the same float-rounded
+# links.  Measured: force 4.7e-6 ... 6.1e-6 of its maximum, dH within 7e-7 ... 3.8e-5; bars 2e-5
+# and 1e-4.
+@pytest.mark.parametrize("dims", [D4, (12, 4, 4, 4), (2, 4, 2, 6)])
+def test_fp32_bulk_stout_forms(Lat, oracle, dims):
+    L = Lat(dims, precision="f32")
+    U = alg_links(oracle, dims, 0.3, 53)
+    P = gen.gaussian_momenta(dims, 54)
+    L.upload_links(U)
+    U = L.download_links()  # the float-rounded links, for both sides
+    L.stout_config(3, 0.1, "store")
+    L.force(COEFF)
+    F = oracle.stout_force(dims, U, 3, 0.1, COEFF)
+    assert maxdiff(L.download_force(), F) < 2e-5 * np.abs(F).max()
+    L.upload_momenta(P)
+    P = L.download_momenta()
+    H = L.md_trajectory(COEFF, 4, 0.2, "2MN", fused=True)
+    Href = oracle.md_trajectory(dims, U.copy(), P.copy(), COEFF, 4, 0.2, "2MN", nlev=3, rho=0.1)
+    assert abs(H[2] - Href[2]) < 1e-4
+
+
 # ------------------------------------------------------------------ workgroup orderings
 # Extents whose xy-plane holds a multiple of 64 sites run the stencil kernels in a non-linear
 # group order: the t-sweep tiles when the (xy-group, z) plane splits into 8 tiles ((8,8,8,8),
