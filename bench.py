@@ -332,6 +332,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-stout", action="store_true", help="skip the stout-HMC s/traj lines")
     ap.add_argument("--stout-therm", type=int, default=6, help="FP64 thermalization trajectories")
+    ap.add_argument("--stout-draws", type=int, default=8,
+                    help="momentum draws of the FP64 vs FP32-bulk dH comparison")
     ap.add_argument("--stout-only", action="store_true", help="run only the stout-HMC and flow comparisons")
     ap.add_argument("--no-next", action="store_true", help="skip the heatbath / random-field / field-strength / ILDG timings")
     ap.add_argument("--next-only", action="store_true",
@@ -355,7 +357,7 @@ def main():
         print(json.dumps({"next_rows": next_rows(gf)}))
         return
     if args.stout_only:  # profiling aid: only the stout-HMC and flow comparisons
-        print(json.dumps({"stout_hmc": stout_hmc_thermalized(gf, ntherm=args.stout_therm),
+        print(json.dumps({"stout_hmc": stout_hmc_thermalized(gf, ntherm=args.stout_therm, npairs=args.stout_draws),
                           "wilson_flow": flow_compare(gf, steps=30)}))
         return
     V = int(np.prod(DIMS))
@@ -416,7 +418,7 @@ def main():
     stout = None
     flow = None
     if not args.no_stout:
-        stout = stout_hmc_thermalized(gf, ntherm=args.stout_therm)
+        stout = stout_hmc_thermalized(gf, ntherm=args.stout_therm, npairs=args.stout_draws)
         log(f"[bench] stout {json.dumps(stout)}")
         flow = flow_compare(gf)
         log(f"[bench] flow {json.dumps(flow)}")
