@@ -108,6 +108,14 @@ void release(gf_ctx* c, void** p, size_t n) {
         if (_r != GF_OK) return _r;     \
     } while (0)
 
+// Largest byte extent of one buffer descriptor (one direction of one field) a context of V
+// sites creates.  FP64: a general matrix field, 9 complex planes of 16 B per site.  FP32: the
+// general matrix field (4 quad planes + 1 pair plane, 72 B per site) and the FP64 link fields
+// the Hamiltonian widens the links into (6 planes of 16 B, 96 B per site).
+long long max_descriptor_bytes(long long V, int precision) {
+    return precision == GF_F64 ? 144 * V : std::max(72 * V, 96 * V);
+}
+
 int need_staging(gf_ctx* c) { return alloc(c, (void**)&c->staging, (size_t)72 * c->V * sizeof(double)); }
 
 // typed dispatch: f.template operator()<T>()
@@ -319,9 +327,9 @@ int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
         if (dims[m] <= 0) return GF_EINVAL;
         V *= dims[m];
     }
-    // one direction of one field (9 complex planes of V sites) is one buffer descriptor whose
-    // byte extent must fit the descriptor's 32-bit record count
-    if (V * 9 * 2 * (precision == GF_F64 ? 8 : 4) > 0xFFFFFFFFLL) return GF_EINVAL;
+    // one direction of one field is one buffer descriptor whose byte extent must fit the
+    // descriptor's 32-bit record count
+    if (max_descriptor_bytes(V, precision) > 0xFFFFFFFFLL) return GF_EINVAL;
     if (hipSetDevice(device) != hipSuccess) return GF_EDEVICE;
     gf_ctx* c = new gf_ctx();
     for (int m = 0; m < 4; m++) c->dims[m] = dims[m];
@@ -465,35 +473,61 @@ int gf_set_cold(gf_ctx* c) {
 }
 
 // The device keeps rows 0 and 1 of each link and rebuilds row 2 as conj(row0 x row1), which
-// holds for SU(3).  Links whose third row differs from that by more than kSu3Tol are rejected.
+// holds for SU(3) (unitary, det 1) and for nothing else.  A link is accepted if rows 0 and 1
+// are orthonormal (every entry of their 2x2 Gram matrix within kSu3Tol of the identity's) and
+// row 2 equals conj(row0 x row1) to kSu3Tol (sum of |re| and |im| differences per entry);
+// together these bound max |U U^dag - 1| and |det U - 1| by a small multiple of kSu3Tol.  The
+// limit admits links stored in 32-bit precision (ILDG files with 32-bit payloads).
 constexpr double kSu3Tol = 1e-6;
-static double su3_row2_deviation(const double* ref, size_t nlinks) {
-    double worst = 0.0;
+struct Su3Check {
+    double worst = 0.0;  // largest deviation found (NaN if a NaN was met)
+    size_t link = 0;     // link index (mu * V + site) of the largest deviation
+    const char* what = "";
+};
+static Su3Check su3_deviation(const double* ref, size_t nlinks) {
+    Su3Check r;
     for (size_t l = 0; l < nlinks; l++) {
         const double* m = ref + 18 * l;  // element (i, j) at complex offset i + 3 j
         auto re = [&](int i, int j) { return m[2 * (i + 3 * j)]; };
         auto im = [&](int i, int j) { return m[2 * (i + 3 * j) + 1]; };
+        auto note = [&](double d, const char* what) {
+            if (std::isnan(d) || d > r.worst) {
+                r.worst = d;
+                r.link = l;
+                r.what = what;
+            }
+        };
+        // rows 0 and 1: norms and overlap
+        double n0 = 0.0, n1 = 0.0, pr = 0.0, pi = 0.0;
+        for (int j = 0; j < 3; j++) {
+            n0 += re(0, j) * re(0, j) + im(0, j) * im(0, j);
+            n1 += re(1, j) * re(1, j) + im(1, j) * im(1, j);
+            pr += re(0, j) * re(1, j) + im(0, j) * im(1, j);  // row0 . conj(row1)
+            pi += im(0, j) * re(1, j) - re(0, j) * im(1, j);
+        }
+        note(std::fabs(n0 - 1.0), "row 0 is not of unit norm");
+        note(std::fabs(n1 - 1.0), "row 1 is not of unit norm");
+        note(std::hypot(pr, pi), "rows 0 and 1 are not orthogonal");
         for (int k = 0; k < 3; k++) {
             const int a = (k + 1) % 3, b = (k + 2) % 3;
             const double cr = (re(0, a) * re(1, b) - im(0, a) * im(1, b)) - (re(0, b) * re(1, a) - im(0, b) * im(1, a));
             const double ci = (re(0, a) * im(1, b) + im(0, a) * re(1, b)) - (re(0, b) * im(1, a) + im(0, b) * re(1, a));
-            const double d = std::fabs(re(2, k) - cr) + std::fabs(im(2, k) + ci);
-            if (std::isnan(d)) return d;
-            if (d > worst) worst = d;
+            note(std::fabs(re(2, k) - cr) + std::fabs(im(2, k) + ci), "row 2 differs from conj(row0 x row1)");
         }
+        if (std::isnan(r.worst)) return r;
     }
-    return worst;
+    return r;
 }
 
 int gf_upload_links(gf_ctx* c, const double* ref) {
     CTXCHK(c);
     if (!ref) return fail(c, GF_EINVAL, "null links");
     {
-        const double dev = su3_row2_deviation(ref, (size_t)4 * c->V);
-        if (!(dev <= kSu3Tol))
+        const Su3Check chk = su3_deviation(ref, (size_t)4 * c->V);
+        if (!(chk.worst <= kSu3Tol))
             return fail(c, GF_EINVAL,
-                        "links must be SU(3): the third row differs from conj(row1 x row2) by %g (limit %g)", dev,
-                        kSu3Tol);
+                        "links must be SU(3): at direction %d, site %zu %s by %g (limit %g)",
+                        (int)(chk.link / (size_t)c->V) + 1, chk.link % (size_t)c->V, chk.what, chk.worst, kSu3Tol);
     }
     TRY(need_staging(c));
     HIPCHK(c, hipMemcpyAsync(c->staging, ref, (size_t)72 * c->V * sizeof(double), hipMemcpyHostToDevice, c->stream));

@@ -133,28 +133,10 @@ struct C2 {
 };
 __device__ __forceinline__ C2 c2(double r, double i) { return C2{r, i}; }
 __device__ __forceinline__ C2 cmul2(C2 a, C2 b) { return C2{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
-__device__ __forceinline__ C2 cconj2(C2 a) { return C2{a.r, -a.i}; }
 __device__ __forceinline__ C2 cadd2(C2 a, C2 b) { return C2{a.r + b.r, a.i + b.i}; }
-__device__ __forceinline__ C2 cscale2(double s, C2 a) { return C2{s * a.r, s * a.i}; }
-__device__ __forceinline__ C2 cdiv2(C2 a, C2 b) {
-    const double d = b.r * b.r + b.i * b.i;
-    return C2{(a.r * b.r + a.i * b.i) / d, (a.i * b.r - a.r * b.i) / d};
-}
 struct M2 {
     C2 e[2][2];
 };
-
-// project_onto_SU2! (heatbath/portable/kernels.jl:12-23)
-__device__ __forceinline__ M2 su2_project(const M2& S) {
-    const C2 a = cadd2(cscale2(0.5, S.e[0][0]), cscale2(0.5, cconj2(S.e[1][1])));
-    const C2 b = cadd2(cscale2(0.5, S.e[1][0]), cscale2(-0.5, cconj2(S.e[0][1])));
-    M2 r;
-    r.e[0][0] = a;
-    r.e[1][0] = b;
-    r.e[0][1] = cscale2(-1.0, cconj2(b));
-    r.e[1][1] = cconj2(a);
-    return r;
-}
 
 template <int n, int k>
 __device__ __forceinline__ M2 sub2(const M3<double>& m) {
@@ -181,82 +163,83 @@ __device__ __forceinline__ void su2_left_mul(const M2& K, M3<double>& u) {
     }
 }
 
-// Kennedy-Pendleton SU(2) heatbath (_su2_update_kp_core!, heatbath/portable/kernels.jl:63-150)
+// SU(2) elements (times a real scale) as quaternions: q = (q0, q1, q2, q3) stands for
+// X(q) = q0 + i (q1 s1 + q2 s2 + q3 s3) with the Pauli matrices s_k, i.e.
+// X = [[q0 + i q3, q2 + i q1], [-q2 + i q1, q0 - i q3]];  det X = |q|^2, X(q)^dag = X(conj q),
+// X(p) X(q) = X(p q) with the Hamilton product below.
+struct Quat {
+    double q0, q1, q2, q3;
+};
+__device__ __forceinline__ Quat qmul(const Quat& p, const Quat& q) {
+    // (p0 q0 - p.q,  p0 q + q0 p - p x q)
+    return Quat{p.q0 * q.q0 - (p.q1 * q.q1 + p.q2 * q.q2 + p.q3 * q.q3),
+                p.q0 * q.q1 + q.q0 * p.q1 - (p.q2 * q.q3 - p.q3 * q.q2),
+                p.q0 * q.q2 + q.q0 * p.q2 - (p.q3 * q.q1 - p.q1 * q.q3),
+                p.q0 * q.q3 + q.q0 * p.q3 - (p.q1 * q.q2 - p.q2 * q.q1)};
+}
+__device__ __forceinline__ Quat qconj(const Quat& q) { return Quat{q.q0, -q.q1, -q.q2, -q.q3}; }
+__device__ __forceinline__ double qnorm2(const Quat& q) { return q.q0 * q.q0 + q.q1 * q.q1 + q.q2 * q.q2 + q.q3 * q.q3; }
+__device__ __forceinline__ Quat qscale(double s, const Quat& q) { return Quat{s * q.q0, s * q.q1, s * q.q2, s * q.q3}; }
+// the quaternion part of a general 2x2 complex block: its projection onto the real span of SU(2)
+// (project_onto_SU2!, heatbath/portable/kernels.jl:12-23, forms the same element as a matrix)
+__device__ __forceinline__ Quat quat_of(const M2& S) {
+    return Quat{0.5 * (S.e[0][0].r + S.e[1][1].r), 0.5 * (S.e[0][1].i + S.e[1][0].i),
+                0.5 * (S.e[0][1].r - S.e[1][0].r), 0.5 * (S.e[0][0].i - S.e[1][1].i)};
+}
+__device__ __forceinline__ M2 mat_of(const Quat& q) {
+    M2 m;
+    m.e[0][0] = c2(q.q0, q.q3);
+    m.e[0][1] = c2(q.q2, q.q1);
+    m.e[1][0] = c2(-q.q2, q.q1);
+    m.e[1][1] = c2(q.q0, -q.q3);
+    return m;
+}
+// q / |q|: keeps a product of unit quaternions on SU(2) against rounding
+__device__ __forceinline__ Quat qunit(const Quat& q) { return qscale(1.0 / sqrt(qnorm2(q)), q); }
+
+// SU(2) heatbath of one Cabibbo-Marinari subgroup (N. Cabibbo, E. Marinari, Phys. Lett. B 119
+// (1982) 387), the sampler of A. D. Kennedy, B. J. Pendleton, Phys. Lett. B 156 (1985) 393.
+// The action restricted to the subgroup element X is (beta/3) Re tr(X W) with W the subgroup
+// block of u Sigma, whose quaternion part is w = |w| v, v in SU(2).  X = X0 v^dag with X0
+// drawn from the density sqrt(1 - x0^2) exp(k x0) dx0 dOmega, k = 2 (beta/3) |w|:
+// delta = 1 - x0 is Gamma(3/2, 1/k) distributed, sampled as -ln(r1)/k - ln(r2)/k cos^2(2 pi r3)
+// and accepted with probability sqrt(1 - delta/2) (test r4^2 <= 1 - delta/2); the direction of
+// the vector part is uniform on the sphere.  (The reference's steps are the same published
+// algorithm, heatbath/portable/kernels.jl:63-150; they use this stream's draws in this order.)
+// Returns false (no subgroup update) if no proposal is accepted within itmax tries.
 __device__ __forceinline__ bool kp_update(unsigned long long seed, unsigned long long sweep, unsigned s, int mu,
-                                          int tag, double beta, int itmax, const M2& S, M2& K) {
-    const double rho0 = (S.e[0][0].r + S.e[1][1].r) * 0.5, rho1 = -(S.e[0][1].i + S.e[1][0].i) * 0.5;
-    const double rho2 = (S.e[1][0].r - S.e[0][1].r) * 0.5, rho3 = (S.e[1][1].i - S.e[0][0].i) * 0.5;
-    const double rho = sqrt(rho0 * rho0 + rho1 * rho1 + rho2 * rho2 + rho3 * rho3);
-    const C2 det = cadd2(cmul2(S.e[0][0], S.e[1][1]), cscale2(-1.0, cmul2(S.e[0][1], S.e[1][0])));
-    M2 V0;
-    V0.e[0][0] = cdiv2(cscale2(rho, S.e[1][1]), det);
-    V0.e[0][1] = cdiv2(cscale2(-rho, S.e[0][1]), det);
-    V0.e[1][0] = cdiv2(cscale2(-rho, S.e[1][0]), det);
-    V0.e[1][1] = cdiv2(cscale2(rho, S.e[0][0]), det);
-    const double k = 2.0 * (beta / 3.0) * rho;
+                                          int tag, double beta, int itmax, const Quat& w, Quat& X) {
+    const double wn = sqrt(qnorm2(w));
+    const double k = 2.0 * (beta / 3.0) * wn;
     double delta = 0.0;
     bool ok = false;
     for (int t = 0; t < itmax && !ok; t++) {
-        unsigned w[4];
-        link_block(seed, sweep, s, mu, tag, t, w);
-        const double X = -log(unit32(w[0])) / k, Xp = -log(unit32(w[1])) / k;
-        const double c = cos(2.0 * 3.14159265358979323846 * unit32(w[2]));
-        delta = Xp + X * c * c;
-        const double r3 = unit32(w[3]);
-        ok = r3 * r3 <= 1.0 - 0.5 * delta;
+        unsigned r[4];
+        link_block(seed, sweep, s, mu, tag, t, r);
+        const double c = cos(2.0 * 3.14159265358979323846 * unit32(r[2]));
+        delta = -log(unit32(r[1])) / k + (-log(unit32(r[0])) / k) * c * c;
+        const double r4 = unit32(r[3]);
+        ok = r4 * r4 <= 1.0 - 0.5 * delta;
     }
     if (!ok) return false;
-    unsigned w[4];
-    link_block(seed, sweep, s, mu, tag, 0xFFFF, w);
-    const double a1 = 1.0 - delta, rr = sqrt(1.0 - a1 * a1);
-    const double phi = 2.0 * 3.14159265358979323846 * unit32(w[0]);
-    const double ct = 2.0 * (unit32(w[1]) - 0.5), st = sqrt(1.0 - ct * ct);
-    const double a2 = rr * cos(phi) * st, a3 = rr * sin(phi) * st, a4 = rr * ct;
-    M2 tmp;
-    tmp.e[0][0] = c2(a1, a4);
-    tmp.e[0][1] = c2(a3, a2);
-    tmp.e[1][0] = c2(-a3, a2);
-    tmp.e[1][1] = c2(a1, -a4);
-    M2 Un;
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) Un.e[i][j] = cadd2(cmul2(tmp.e[i][0], V0.e[0][j]), cmul2(tmp.e[i][1], V0.e[1][j]));
-    const C2 al = cadd2(cscale2(0.5, Un.e[0][0]), cscale2(0.5, cconj2(Un.e[1][1])));
-    const C2 be = cadd2(cscale2(0.5, Un.e[1][0]), cscale2(-0.5, cconj2(Un.e[0][1])));
-    const double d = al.r * al.r + al.i * al.i + be.r * be.r + be.i * be.i;
-    K.e[0][0] = cscale2(1.0 / d, al);
-    K.e[1][0] = cscale2(1.0 / d, be);
-    K.e[0][1] = cscale2(-1.0 / d, cconj2(be));
-    K.e[1][1] = cscale2(1.0 / d, cconj2(al));
+    unsigned r[4];
+    link_block(seed, sweep, s, mu, tag, 0xFFFF, r);
+    const double x0 = 1.0 - delta, len = sqrt(1.0 - x0 * x0);
+    const double phi = 2.0 * 3.14159265358979323846 * unit32(r[0]);
+    const double cth = 2.0 * (unit32(r[1]) - 0.5), sth = sqrt(1.0 - cth * cth);
+    const Quat X0{x0, len * sth * cos(phi), len * sth * sin(phi), len * cth};
+    X = qunit(qmul(X0, qscale(1.0 / wn, qconj(w))));  // X0 v^dag
     return true;
 }
 
-// overrelaxation subgroup step (_overrelaxation_subgroup!, heatbathmodule.jl:1248-1280)
+// Overrelaxation of one subgroup (M. Creutz, Phys. Rev. D 36 (1987) 515; F. R. Brown,
+// T. J. Woch, Phys. Rev. Lett. 58 (1987) 2394): with v the SU(2) direction of the subgroup block
+// of u Sigma, X = (v^dag)^2 maps Re tr(u Sigma) restricted to the subgroup onto itself
+// (heatbathmodule.jl:1248-1280 applies the same reflection).
 template <int n, int k>
 __device__ __forceinline__ void or_subgroup(M3<double>& u, const M3<double>& V) {
-    const M2 w = su2_project(sub2<n, k>(mul(u, V)));
-    M2 h;
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-            h.e[i][j] = cadd2(cmul2(cconj2(w.e[0][i]), cconj2(w.e[j][0])), cmul2(cconj2(w.e[1][i]), cconj2(w.e[j][1])));
-    // Gram-Schmidt on the columns
-    {
-        const double n0 = 1.0 / sqrt(h.e[0][0].r * h.e[0][0].r + h.e[0][0].i * h.e[0][0].i + h.e[1][0].r * h.e[1][0].r +
-                                     h.e[1][0].i * h.e[1][0].i);
-        h.e[0][0] = cscale2(n0, h.e[0][0]);
-        h.e[1][0] = cscale2(n0, h.e[1][0]);
-        const C2 p = cadd2(cmul2(cconj2(h.e[0][0]), h.e[0][1]), cmul2(cconj2(h.e[1][0]), h.e[1][1]));
-        h.e[0][1] = cadd2(h.e[0][1], cscale2(-1.0, cmul2(p, h.e[0][0])));
-        h.e[1][1] = cadd2(h.e[1][1], cscale2(-1.0, cmul2(p, h.e[1][0])));
-        const double n1 = 1.0 / sqrt(h.e[0][1].r * h.e[0][1].r + h.e[0][1].i * h.e[0][1].i + h.e[1][1].r * h.e[1][1].r +
-                                     h.e[1][1].i * h.e[1][1].i);
-        h.e[0][1] = cscale2(n1, h.e[0][1]);
-        h.e[1][1] = cscale2(n1, h.e[1][1]);
-    }
-    su2_left_mul<n, k>(h, u);
+    const Quat vd = qunit(qconj(quat_of(sub2<n, k>(mul(u, V)))));
+    su2_left_mul<n, k>(mat_of(qunit(qmul(vd, vd))), u);
 }
 
 // one heatbath subgroup step; false (u unchanged) if no proposal was accepted
@@ -264,9 +247,9 @@ template <int n, int k>
 __device__ __forceinline__ bool hb_subgroup(M3<double>& u, const M3<double>& St, unsigned long long seed,
                                             unsigned long long sweep, unsigned s, int mu, int tag, double beta,
                                             int itmax) {
-    M2 K;
-    if (!kp_update(seed, sweep, s, mu, tag, beta, itmax, su2_project(sub2<n, k>(mul(u, St))), K)) return false;
-    su2_left_mul<n, k>(K, u);
+    Quat X;
+    if (!kp_update(seed, sweep, s, mu, tag, beta, itmax, quat_of(sub2<n, k>(mul(u, St))), X)) return false;
+    su2_left_mul<n, k>(mat_of(X), u);
     return true;
 }
 

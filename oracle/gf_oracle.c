@@ -824,12 +824,21 @@ void gfo_md_trajectory(const int *dims, double *U, double *P, double coeff, int 
             md_P(&m, U, P, eps / 2);
             md_Q(&m, U, P, eps);
             md_P(&m, U, P, eps / 2);
-        } else {
+        } else if (integrator == 2) {
             md_P(&m, U, P, lambda * eps);
             md_Q(&m, U, P, 0.5 * eps);
             md_P(&m, U, P, (1.0 - 2.0 * lambda) * eps);
             md_Q(&m, U, P, 0.5 * eps);
             md_P(&m, U, P, lambda * eps);
+        } else {
+            /* 2MN with the closing kick of a step and the opening kick of the next one applied
+             * as one kick of 2 lambda eps: the links do not change between the two, so the
+             * sequence is the same map; one force evaluation per step fewer */
+            md_P(&m, U, P, (k == 0 ? 1.0 : 2.0) * lambda * eps);
+            md_Q(&m, U, P, 0.5 * eps);
+            md_P(&m, U, P, (1.0 - 2.0 * lambda) * eps);
+            md_Q(&m, U, P, 0.5 * eps);
+            if (k == steps - 1) md_P(&m, U, P, lambda * eps);
         }
     }
     double h1 = md_H(&m, U, P);

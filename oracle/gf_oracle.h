@@ -77,7 +77,8 @@ long gfo_heatbath_sweep(const int *dims, double *U, double beta, uint64_t seed, 
 /* Kennedy-Pendleton SU(2) samples: a0 of n elements at coupling k */
 void gfo_kp_samples(double k, long n, uint64_t seed, double *a0);
 
-/* integrator: 0 = QPQ, 1 = PQP, 2 = Omelyan 2MN (lambda argument) */
+/* integrator: 0 = QPQ, 1 = PQP, 2 = Omelyan 2MN (lambda argument), 3 = 2MN with the adjacent
+ * kicks of consecutive steps merged into one (the same map, one force evaluation fewer per step) */
 void gfo_md_trajectory(const int *dims, double *U, double *P, double coeff, int nlev,
                        double rho, int integrator, double lambda, int steps, double tau,
                        double *H);

@@ -231,7 +231,9 @@ def stout_potential(dims, U, nlev, rho, coeff):
     return lib().gfo_stout_potential(_dims(dims), _p(_links(U)), int(nlev), float(rho), float(coeff))
 
 
-INTEGRATORS = {"QPQ": 0, "PQP": 1, "2MN": 2}
+# "2MN-merged": the Omelyan 2MN sequence with the adjacent kicks of consecutive steps applied as
+# one kick (the same map; one force evaluation per step fewer)
+INTEGRATORS = {"QPQ": 0, "PQP": 1, "2MN": 2, "2MN-merged": 3}
 
 
 def md_trajectory(dims, U, P, coeff, steps, tau, integrator="QPQ", lam=0.1931833275037836,

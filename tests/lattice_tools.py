@@ -132,6 +132,84 @@ def perturb_link(U, mu, s, a, h):
     return gen.matrices_to_links(M)
 
 
+def stout_force_fd_local(oracle, dims, U, nlev, rho, coeff, mu, s, a, h):
+    """Central finite difference -(V(+h) - V(-h)) / 2h of the stout-smeared Wilson potential
+    V = -(2 coeff / 3) sum Re tr P(smeared) under U_mu(s) -> exp(+-h i T_a) U_mu(s), evaluated
+    locally so that its rounding stays far below the O(h^2) truncation even on large lattices.
+
+    Changing one thin link changes the level-l smeared links only within max-norm distance l of
+    its site, hence the plaquettes only at base sites within nlev + 1.  The smeared links there
+    are exact on a periodic window of the lattice covering offsets -(nlev+4) .. nlev+5 around the
+    site (the wrap-around at the window's faces corrupts one layer per level from outside in), so
+    each evaluation smears that window with the oracle and sums the per-plaquette differences
+    tr P(+h) - tr P(-h) over the base sites within nlev + 1 (all other differences are exactly
+    zero).  Axes not longer than the window are taken whole (exactly periodic)."""
+    NX, NY, NZ, NT = dims
+    x = [s % NX, (s // NX) % NY, (s // (NX * NY)) % NZ, s // (NX * NY * NZ)]
+    lo, hi = nlev + 4, nlev + 5
+    W = field(U, dims)
+    wdims, cen, box = [], [], []
+    for d in range(4):
+        n = dims[d]
+        if n <= lo + hi + 1:  # the whole axis, rolled so that the site sits at index n // 2
+            c = n // 2
+            idx = (x[d] + np.arange(n) - c) % n
+            box.append((0, n))
+        else:
+            c = lo
+            idx = (x[d] + np.arange(-lo, hi + 1)) % n
+            box.append((c - nlev - 1, c + nlev + 2))
+        W = np.take(W, idx, axis=_AX[d] + 1)
+        wdims.append(len(idx))
+        cen.append(c)
+    wdims = tuple(wdims)
+    site = (cen[3], cen[2], cen[1], cen[0])
+    tr = []
+    for sgn in (1, -1):
+        Wp = W.copy()
+        Wp[(mu,) + site] = expm(sgn * h * 1j * T_GEN[a]) @ W[(mu,) + site]
+        S = field(oracle.stout_smear(wdims, np.ascontiguousarray(unfield(Wp)), nlev, rho), wdims)
+        t = 0.0
+        for m in range(4):
+            for n_ in range(m + 1, 4):
+                P = S[m] @ sh(S[n_], m) @ dg(sh(S[m], n_)) @ dg(S[n_])
+                sl = tuple(slice(*box[d]) for d in (3, 2, 1, 0))
+                t = t + np.trace(P[sl], axis1=-2, axis2=-1).real
+        tr.append(t)
+    dV = -(2.0 * coeff / 3.0) * float(np.sum(tr[0] - tr[1]))
+    return -dV / (2 * h)
+
+
+def config1_chain(oracle, run_gpu=None, n_therm=10, n_traj=20, coeff=3.0):
+    """The plain-HMC workload on 4^4 (beta 6, QPQ 10 steps, tau 1) from the Gram-Schmidt hot
+    start of seed 101: n_therm trajectories accepted unconditionally, then n_traj with the
+    Metropolis test (uniforms from Philox stream 103); momenta from Philox stream 102.  The
+    oracle runs the chain; run_gpu(U0), if given, receives the start links and returns a callback
+    step(P, r, metropolis, record) that the GPU side runs after each oracle trajectory.
+    Returns the oracle's per-trajectory records (H, accepted, plaquette sum after it)."""
+    dims = (4, 4, 4, 4)
+    Uo = gen.hot_start_links(dims, 101)
+    rng_mom = np.random.Generator(np.random.Philox(102))
+    rng_acc = np.random.Generator(np.random.Philox(103))
+    step = run_gpu(Uo.copy()) if run_gpu else None
+    recs = []
+    for traj in range(n_therm + n_traj):
+        P = rng_mom.standard_normal((4, 256, 8))
+        metropolis = traj >= n_therm
+        r = rng_acc.random() if metropolis else 0.0
+        Ut, Pt = Uo.copy(), P.copy()
+        Href = oracle.md_trajectory(dims, Ut, Pt, coeff, 10, 1.0, "QPQ")
+        acc = bool((not metropolis) or np.exp(-Href[2]) >= r)
+        if acc:
+            Uo = Ut
+        rec = {"traj": traj, "H": Href, "accepted": acc, "metropolis": metropolis,
+               "plaquette_sum": oracle.plaquette_sum(dims, Uo)}
+        if step:
+            step(P, r, metropolis, rec)
+        recs.append(rec)
+    return recs
+
+
 # ---------------------------------------------------------------- field strengths
 
 

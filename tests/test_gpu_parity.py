@@ -71,6 +71,28 @@ def test_upload_rejects_non_su3(Lat, gf):
     M[1, 17, 2, 0] = np.nan
     with pytest.raises(gf.GFError, match="links must be SU"):
         L.upload_links(gen.matrices_to_links(M))
+
+    def with_row2(M):  # row 2 = conj(row0 x row1): passes the row-2 test whatever rows 0, 1 are
+        M = M.copy()
+        M[..., 2, :] = np.conj(np.cross(M[..., 0, :], M[..., 1, :]))
+        return gen.matrices_to_links(M)
+
+    M = gen.links_to_matrices(U).copy()
+    M[2, 5, 0, :] *= 1.0 + 1e-4  # row 0 not of unit norm
+    with pytest.raises(gf.GFError, match="direction 3, site 5 row 0 is not of unit norm"):
+        L.upload_links(with_row2(M))
+    M = gen.links_to_matrices(U).copy()
+    M[0, 40, 1, :] += 1e-4 * M[0, 40, 0, :]  # rows 0 and 1 not orthogonal
+    with pytest.raises(gf.GFError, match="direction 1, site 40 rows 0 and 1 are not orthogonal"):
+        L.upload_links(with_row2(M))
+    # a non-unitary link is rejected and the configuration on the device is left as it was
+    L.upload_links(U)
+    bad = gen.links_to_matrices(U).copy()
+    bad[3, 7] *= 1.01
+    with pytest.raises(gf.GFError, match="links must be SU"):
+        L.upload_links(gen.matrices_to_links(bad))
+    assert np.array_equal(gen.links_to_matrices(L.download_links())[..., :2, :],
+                          gen.links_to_matrices(U)[..., :2, :])
     # within the limit (links written in single precision) is accepted
     L.upload_links(U.astype(np.complex64).astype(np.complex128))
 

@@ -354,6 +354,34 @@ def test_md_dH_scales_as_eps_squared(oracle):
     assert 3.0 < dH[0] / dH[1] < 5.0
 
 
+@pytest.mark.parametrize("nlev", [0, 2])
+def test_md_2mn_merged_kicks_is_the_same_map(oracle, nlev):
+    # adjacent kicks of consecutive 2MN steps act on the same links: merging them changes the
+    # result by rounding only, while a dropped or doubled kick would move P by O(eps)
+    U0 = alg_links(oracle, D4, 0.3, 97)
+    P0 = gen.gaussian_momenta(D4, 98)
+    out = {}
+    for integ in ("2MN", "2MN-merged"):
+        U, P = U0.copy(), P0.copy()
+        H = oracle.md_trajectory(D4, U, P, COEFF, 4, 0.5, integ, nlev=nlev, rho=0.1)
+        out[integ] = (U, P, H)
+    (Ua, Pa, Ha), (Ub, Pb, Hb) = out["2MN"], out["2MN-merged"]
+    assert np.abs(Ua - Ub).max() < 1e-13 and np.abs(Pa - Pb).max() < 1e-12
+    assert abs(Ha[2] - Hb[2]) < 1e-12 * abs(Ha[0])
+    assert np.abs(Pa - P0).max() > 1e-2
+
+
+def test_config1_chain_accepts(oracle):
+    # the plain-HMC workload after its thermalization: the Metropolis trajectories that the GPU
+    # test compares are mostly accepted, so the accept-and-evolve path is what gets compared
+    recs = lt.config1_chain(oracle)
+    dH = [r["H"][2] for r in recs]
+    assert dH[0] > 5.0  # hot start: far from equilibrium
+    n_acc = sum(r["accepted"] for r in recs if r["metropolis"])
+    assert n_acc >= 10, (n_acc, dH)
+    assert all(abs(d) < 2.0 for d in dH[10:]), dH
+
+
 def test_md_stout_reversibility(oracle):
     U0 = alg_links(oracle, D4, 0.3, 95)
     P0 = gen.gaussian_momenta(D4, 96)
