@@ -3,6 +3,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <algorithm>
@@ -114,6 +115,81 @@ void release(gf_ctx* c, void** p, size_t n) {
 // the Hamiltonian widens the links into (6 planes of 16 B, 96 B per site).
 long long max_descriptor_bytes(long long V, int precision) {
     return precision == GF_F64 ? 144 * V : std::max(72 * V, 96 * V);
+}
+
+// Site geometry of extents dims (V sites): strides, divisor constants and the order in which
+// workgroups visit the 64-site groups.  `seen` (at least V / 64 bytes) is the work space of the
+// host check that the t-sweep block -> group map is a permutation.
+void make_geometry(const int* dims, long long V, Geo& g, char* seen) {
+    g = Geo{};
+    g.V = (int)V;
+    int st = 1;
+    for (int m = 0; m < 4; m++) {
+        g.n[m] = dims[m];
+        g.st[m] = st;
+        st *= dims[m];
+    }
+    set_divisors(g);
+    // Morton order of the 64-site groups over (xy-group, z, t) when every extent is a power of two
+    {
+        auto log2_exact = [](long long n) -> int {
+            if (n <= 0 || (n & (n - 1))) return -1;
+            int b = 0;
+            while ((1LL << b) < n) b++;
+            return b;
+        };
+        const long long nxy = (long long)dims[0] * dims[1];
+        const int bxy = nxy % 64 == 0 ? log2_exact(nxy / 64) : -1;
+        const int bz = log2_exact(dims[2]), bt = log2_exact(dims[3]);
+        if (bxy >= 0 && bz >= 0 && bt >= 0) {
+            g.mbits[0] = bxy;
+            g.mbits[1] = bz;
+            g.mbits[2] = bt;
+        } else {
+            g.mbits[0] = -1;
+            g.mbits[1] = g.mbits[2] = 0;
+        }
+        // t-sweep tiles: 8 tiles of cy x cz (xy-group, z) columns, each split along z into `sub`
+        // sub-tiles swept in t one after another.  The choice targets sub-tiles of about 16
+        // columns (1024 sites per t-slice) that are deep in z and y, so that the slices around
+        // the front of a sweep stay in the XCD's L2: at 32^4 (2, 32, 4) fetches 41 % less than
+        // the unsplit 4 x 16 tiles (rocprof FETCH_SIZE) and runs 3 % faster.
+        g.tile[0] = g.tile[1] = 0;
+        g.sub = 1;
+        if (nxy % 64 == 0) {
+            const long long ngy = nxy / 64, nz = dims[2];
+            double best = 1e300;
+            for (long long cy = 1; cy <= ngy; cy++) {
+                if (ngy % cy) continue;
+                for (long long cz = 1; cz <= nz; cz++) {
+                    if (nz % cz || (ngy / cy) * (nz / cz) != 8) continue;
+                    for (long long sb = 1; sb <= cz; sb++) {
+                        if (cz % sb) continue;
+                        const double cols = (double)(cy * (cz / sb));
+                        const double yrows = (double)cy * 64.0 / dims[0], zext = (double)(cz / sb);
+                        const double cost = std::fabs(std::log2(cols / 16.0)) + 1.0 / yrows + 2.0 / zext;
+                        if (cost < best) {
+                            best = cost;
+                            g.tile[0] = (int)cy;
+                            g.tile[1] = (int)cz;
+                            g.sub = (int)sb;
+                        }
+                    }
+                }
+            }
+            if (g.tile[0] > 0) {  // the block -> group map must be a permutation
+                const unsigned nb = (unsigned)(V / 64);
+                std::memset(seen, 0, nb);
+                bool ok = (nb % 8u) == 0;
+                for (unsigned b = 0; ok && b < nb; b++) {
+                    const int q = tsweep_group(g, b, nb);
+                    ok = q >= 0 && (unsigned)q < nb && !seen[q];
+                    if (ok) seen[q] = 1;
+                }
+                if (!ok) g.tile[0] = g.tile[1] = 0;
+            }
+        }
+    }
 }
 
 int need_staging(gf_ctx* c) { return alloc(c, (void**)&c->staging, (size_t)72 * c->V * sizeof(double)); }
@@ -311,12 +387,36 @@ int hamiltonian(gf_ctx* c, double coeff, double* H) {
 
 }  // namespace
 
+
+// ---- plaquette sum of caller-held device links, allocating nothing --------------------------
+// The per-workgroup partial sums of k_plaquette go to `scratch`, a file-scope device array
+// that every device holds once for the life of the process.  Host code never passes the
+// array's name to a kernel: its device address, per device, comes from hipGetSymbolAddress on
+// the first call made with that device current.
+namespace {
+constexpr int kScratchPartials = 1 << 19;  // 4 MiB: one double per 64 sites, V <= 2^25
+__device__ __align__(256) double scratch[kScratchPartials];
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+char g_noalloc_seen[kScratchPartials];  // host work space of make_geometry (under g_noalloc_mu)
+thread_local char g_noalloc_err[512];
+
+int noalloc_fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_noalloc_err, sizeof g_noalloc_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+}  // namespace
+
 // =====================================================================================
 extern "C" {
 
 int gf_version(void) { return GF_VERSION; }
 
-const char* gf_last_error(const gf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* gf_last_error(const gf_ctx* ctx) { return ctx ? ctx->err.c_str() : g_noalloc_err; }
 
 int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
     if (!dims || !out) return GF_EINVAL;
@@ -334,73 +434,9 @@ int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
     gf_ctx* c = new gf_ctx();
     for (int m = 0; m < 4; m++) c->dims[m] = dims[m];
     c->V = (int)V;
-    c->geo.V = (int)V;
-    int st = 1;
-    for (int m = 0; m < 4; m++) {
-        c->geo.n[m] = dims[m];
-        c->geo.st[m] = st;
-        st *= dims[m];
-    }
-    set_divisors(c->geo);
-    // Morton order of the 64-site groups over (xy-group, z, t) when every extent is a power of two
     {
-        auto log2_exact = [](long long n) -> int {
-            if (n <= 0 || (n & (n - 1))) return -1;
-            int b = 0;
-            while ((1LL << b) < n) b++;
-            return b;
-        };
-        const long long nxy = (long long)dims[0] * dims[1];
-        const int bxy = nxy % 64 == 0 ? log2_exact(nxy / 64) : -1;
-        const int bz = log2_exact(dims[2]), bt = log2_exact(dims[3]);
-        if (bxy >= 0 && bz >= 0 && bt >= 0) {
-            c->geo.mbits[0] = bxy;
-            c->geo.mbits[1] = bz;
-            c->geo.mbits[2] = bt;
-        } else {
-            c->geo.mbits[0] = -1;
-            c->geo.mbits[1] = c->geo.mbits[2] = 0;
-        }
-        // t-sweep tiles: 8 tiles of cy x cz (xy-group, z) columns, each split along z into `sub`
-        // sub-tiles swept in t one after another.  The choice targets sub-tiles of about 16
-        // columns (1024 sites per t-slice) that are deep in z and y, so that the slices around
-        // the front of a sweep stay in the XCD's L2: at 32^4 (2, 32, 4) fetches 41 % less than
-        // the unsplit 4 x 16 tiles (rocprof FETCH_SIZE) and runs 3 % faster.
-        c->geo.tile[0] = c->geo.tile[1] = 0;
-        c->geo.sub = 1;
-        if (nxy % 64 == 0) {
-            const long long ngy = nxy / 64, nz = dims[2];
-            double best = 1e300;
-            for (long long cy = 1; cy <= ngy; cy++) {
-                if (ngy % cy) continue;
-                for (long long cz = 1; cz <= nz; cz++) {
-                    if (nz % cz || (ngy / cy) * (nz / cz) != 8) continue;
-                    for (long long sb = 1; sb <= cz; sb++) {
-                        if (cz % sb) continue;
-                        const double cols = (double)(cy * (cz / sb));
-                        const double yrows = (double)cy * 64.0 / dims[0], zext = (double)(cz / sb);
-                        const double cost = std::fabs(std::log2(cols / 16.0)) + 1.0 / yrows + 2.0 / zext;
-                        if (cost < best) {
-                            best = cost;
-                            c->geo.tile[0] = (int)cy;
-                            c->geo.tile[1] = (int)cz;
-                            c->geo.sub = (int)sb;
-                        }
-                    }
-                }
-            }
-            if (c->geo.tile[0] > 0) {  // the block -> group map must be a permutation
-                const unsigned nb = (unsigned)(V / 64);
-                std::vector<char> seen(nb, 0);
-                bool ok = (nb % 8u) == 0;
-                for (unsigned b = 0; ok && b < nb; b++) {
-                    const int q = tsweep_group(c->geo, b, nb);
-                    ok = q >= 0 && (unsigned)q < nb && !seen[q];
-                    if (ok) seen[q] = 1;
-                }
-                if (!ok) c->geo.tile[0] = c->geo.tile[1] = 0;
-            }
-        }
+        std::vector<char> seen((size_t)(V / 64) + 1);
+        make_geometry(dims, V, c->geo, seen.data());
     }
     c->prec = precision;
     c->esz = precision == GF_F64 ? 8 : 4;
@@ -965,6 +1001,50 @@ int gf_event_elapsed(gf_ctx* c, int a, int b, float* ms) {
     if (a < 0 || a >= GF_NEVENTS || b < 0 || b >= GF_NEVENTS || !ms) return GF_EINVAL;
     HIPCHK(c, hipEventSynchronize(c->ev[b]));
     HIPCHK(c, hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+    return GF_OK;
+}
+
+int gf_plaquette_links(const void* links, const int* dims, int precision, void* stream, double* sum) {
+    g_noalloc_err[0] = 0;
+    if (!links || !dims || !sum) return noalloc_fail(GF_EINVAL, "null links, dims or sum pointer");
+    if (precision != GF_F64 && precision != GF_F32)
+        return noalloc_fail(GF_EINVAL, "precision must be GF_F64 or GF_F32; got %d", precision);
+    long long V = 1;
+    for (int m = 0; m < 4; m++) {
+        if (dims[m] <= 0) return noalloc_fail(GF_EINVAL, "all lattice extents must be positive; got %d", dims[m]);
+        V *= dims[m];
+    }
+    const long long nb = (V + 63) / 64;
+    if (nb > kScratchPartials)
+        return noalloc_fail(GF_EINVAL,
+                            "%lld sites need %lld partial sums; the static scratch holds %d (at most %lld sites)", V,
+                            nb, kScratchPartials, 64LL * kScratchPartials);
+    // one direction of the link field is one buffer descriptor: 6 planes of 16 B (FP64) or 3
+    // quad planes of 16 B (FP32) per site
+    if ((precision == GF_F64 ? 96 : 48) * V > 0xFFFFFFFFLL)
+        return noalloc_fail(GF_EINVAL, "%lld sites exceed the 4 GiB extent of one link direction", V);
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return noalloc_fail(GF_EDEVICE, "hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return noalloc_fail(GF_EDEVICE, "device %d beyond %d", dev, kMaxDevices);
+    Geo g;
+    double* part = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        if (!g_scratch_dev[dev]) {
+            void* p = nullptr;
+            e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return noalloc_fail(GF_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            g_scratch_dev[dev] = (double*)p;
+        }
+        part = g_scratch_dev[dev];
+        make_geometry(dims, V, g, g_noalloc_seen);
+    }
+    const hipStream_t st = (hipStream_t)stream;
+    e = precision == GF_F64 ? launch_plaquette<double>(g, (const double*)links, part, sum, st)
+                            : launch_plaquette<float>(g, (const float*)links, part, sum, st);
+    if (e != hipSuccess) return noalloc_fail(GF_EDEVICE, "plaquette launch: %s", hipGetErrorString(e));
     return GF_OK;
 }
 

@@ -99,6 +99,19 @@ int gf_overrelaxation(gf_ctx *ctx, unsigned long long seed, unsigned long long s
 /* calculate_Plaquette (AbstractGaugefields.jl:2684-2699): sum_{x,mu<nu} Re tr P_munu.
  * measure_plaquette (API.jl:395-404) divides by 6*V*3. */
 int gf_plaquette(gf_ctx *ctx, double *sum);
+/* gf_plaquette for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc); no counterpart in the reference.
+ * links: device pointer to an SU(3) link field of extents dims in the device layout -- FP64:
+ * double[4][6][V][2], plane 6 mu + 3 i + j holding (re, im) of U_mu(x)_ij for rows i = 0, 1;
+ * FP32: float[4][3][V][4], quad plane 3 mu + q holding elements 2q, 2q+1 (element 3 i + j)
+ * of every site -- site s = x + NX (y + NY (z + NZ t)).  sum: device pointer; receives
+ * sum_{x,mu<nu} Re tr P_munu (as gf_plaquette) in stream order on `stream` (a hipStream_t,
+ * NULL for the default stream); the call does not synchronize.  No hipMalloc, hipFree or other
+ * allocation: the per-workgroup partial sums live in a static 4 MiB device array that each GPU
+ * holds until the process ends, so V <= 2^25 sites (larger: GF_EINVAL).  Calls on one GPU
+ * share that array: calls that overlap on one GPU (other streams, other host threads) race.
+ * Errors are reported by gf_last_error(NULL) on the calling thread. */
+int gf_plaquette_links(const void *links, const int *dims, int precision, void *stream, double *sum);
 /* real(evaluate_GaugeAction) (GaugeActions.jl:132-142) of push!(action, coeff,
  * plaquette ∪ plaquette^dag): 2*coeff*sum Re tr P.  md_potential = -value/3. */
 int gf_action(gf_ctx *ctx, double coeff, double *value);
