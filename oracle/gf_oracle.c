@@ -527,6 +527,117 @@ void gfo_exp_update(const int *dims, double *U, const double *P, double eps) {
     lat_free(&L);
 }
 
+/* ------------------------------------------------------------------------- */
+/* the same MD update in the reference's call structure (CPU baseline)         */
+/* ------------------------------------------------------------------------- */
+/* update_momenta! then update_gaugefields! as the reference sequences them
+ * (molecular_dynamics.jl:513-551): md_force! (:251-267) per direction mu runs calc_dSdUμ!
+ * (GaugeActions.jl:95-123), whose six staples evaluate_gaugelinks! forms as whole-lattice
+ * products of (shifted) link fields into temporaries that are then added (add_U!), then
+ * mul!(work, U_mu, dSdU), clear_U!(F_mu) and Traceless_antihermitian_add!(F_mu, -1/NC, work);
+ * add_U!(P_mu, eps, F_mu); then per mu exptU! into a field, mul!(W, E, U_mu) and
+ * substitute_U!(U_mu, W).  Every step is one OpenMP pass over the lattice through full 3x3
+ * temporaries, 104 passes per update: the memory traffic of the reference's structure, not
+ * of the fused restatement above.  Same formulas as gfo_force / gfo_kick / gfo_exp_update. */
+typedef struct {
+    int m1, d1, m2, d2; /* site x + d1 m1-hat + d2 m2-hat (d = 0: no shift) */
+} shift2;
+static inline long shifted(const lat *L, long s, shift2 h) {
+    if (h.d1 > 0) s = UP(L, h.m1, s);
+    if (h.d1 < 0) s = DN(L, h.m1, s);
+    if (h.d2 > 0) s = UP(L, h.m2, s);
+    if (h.d2 < 0) s = DN(L, h.m2, s);
+    return s;
+}
+/* out(x) = opA(A(x + ha)) opB(B(x + hb)), op = identity or ^dag (mode bit 1: A^dag, 2: B^dag) */
+static void pass_mul(const lat *L, double *out, const double *A, shift2 ha, const double *B, shift2 hb, int mode) {
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < L->V; s++) {
+        m3 a = ld(A + 18 * shifted(L, s, ha)), b = ld(B + 18 * shifted(L, s, hb));
+        m3 r = mode == 0 ? mul(a, b) : mode == 1 ? mul_dn(a, b) : mode == 2 ? mul_nd(a, b) : mul(dag(a), dag(b));
+        st(out + 18 * s, r);
+    }
+}
+static void pass_add(const lat *L, double *D, double coeff, const double *T) {
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long i = 0; i < 18 * L->V; i++) D[i] += coeff * T[i];
+}
+static void pass_clear(long n, double *D) {
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long i = 0; i < n; i++) D[i] = 0.0;
+}
+static void pass_ta_add(const lat *L, double *F, double factor, const double *W) {
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < L->V; s++) ta_add(ld(W + 18 * s), factor, F + 8 * s);
+}
+static void pass_kick(long n, double *P, const double *F, double eps) {
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long i = 0; i < n; i++) P[i] += eps * F[i];
+}
+static void pass_exp(const lat *L, double *E, const double *P, double eps) {
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long s = 0; s < L->V; s++) st(E + 18 * s, exp_ta(P + 8 * s, eps));
+}
+static void pass_copy(long n, double *dst, const double *src) {
+#ifdef _OPENMP
+#pragma omp parallel for OMP_NT schedule(static)
+#endif
+    for (long i = 0; i < n; i++) dst[i] = src[i];
+}
+
+void gfo_md_update_unfused(const int *dims, double *U, double *P, double coeff, double eps_p, double eps_u) {
+    lat L = lat_make(dims);
+    const long V = L.V, n18 = 18 * V;
+    double *T1 = (double *)malloc(sizeof(double) * n18), *T2 = (double *)malloc(sizeof(double) * n18);
+    double *D = (double *)malloc(sizeof(double) * n18), *W = (double *)malloc(sizeof(double) * n18);
+    double *F = (double *)malloc(sizeof(double) * 32 * V);
+    const shift2 none = {0, 0, 0, 0};
+    for (int mu = 0; mu < 4; mu++) {
+        double *Umu = U + 18 * (long)mu * V;
+        pass_clear(n18, D);
+        for (int nu = 0; nu < 4; nu++) {
+            if (nu == mu) continue;
+            double *Unu = U + 18 * (long)nu * V;
+            /* U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag */
+            pass_mul(&L, T1, Unu, (shift2){mu, 1, 0, 0}, Umu, (shift2){nu, 1, 0, 0}, 2);
+            pass_mul(&L, T2, T1, none, Unu, none, 2);
+            pass_add(&L, D, coeff, T2);
+            /* U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu) = (U_mu(x-nu) U_nu(x-nu+mu))^dag U_nu(x-nu) */
+            pass_mul(&L, T1, Umu, (shift2){nu, -1, 0, 0}, Unu, (shift2){nu, -1, mu, 1}, 0);
+            pass_mul(&L, T2, T1, none, Unu, (shift2){nu, -1, 0, 0}, 1);
+            pass_add(&L, D, coeff, T2);
+        }
+        pass_mul(&L, W, Umu, none, D, none, 0);
+        pass_clear(8 * V, F + 8 * (long)mu * V);
+        pass_ta_add(&L, F + 8 * (long)mu * V, -1.0 / 3.0, W);
+    }
+    for (int mu = 0; mu < 4; mu++) pass_kick(8 * V, P + 8 * (long)mu * V, F + 8 * (long)mu * V, eps_p);
+    for (int mu = 0; mu < 4; mu++) {
+        double *Umu = U + 18 * (long)mu * V;
+        pass_exp(&L, T1, P + 8 * (long)mu * V, eps_u);
+        pass_mul(&L, W, T1, none, Umu, none, 0);
+        pass_copy(n18, Umu, W);
+    }
+    free(T1);
+    free(T2);
+    free(D);
+    free(W);
+    free(F);
+    lat_free(&L);
+}
+
 /* p*p (TA_gaugefields_4D_serial.jl:106-129): sum of squares of all coefficients;
  * md_hamiltonian uses real(p*p)/2 (molecular_dynamics.jl:494-505). */
 double gfo_momenta_dot(const int *dims, const double *P) {

@@ -44,6 +44,9 @@ void gfo_kick(const int *dims, double *P, const double *F, double eps);
 void gfo_exp_update(const int *dims, double *U, const double *P, double eps);
 double gfo_momenta_dot(const int *dims, const double *P);
 double gfo_potential(const int *dims, const double *U, double coeff);
+/* update_momenta!(eps_p) then update_gaugefields!(eps_u) in the reference's pass structure
+ * (whole-lattice passes through full temporaries; the CPU baseline's reference-structured form) */
+void gfo_md_update_unfused(const int *dims, double *U, double *P, double coeff, double eps_p, double eps_u);
 
 void gfo_flow_force(const int *dims, const double *U, double *F);
 void gfo_flow(const int *dims, double *U, double eps, int nsteps, int every, double *E_out);

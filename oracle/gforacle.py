@@ -41,6 +41,7 @@ def lib():
             "gfo_kick": (None, [_I, _D, _D, ctypes.c_double]),
             "gfo_exp_update": (None, [_I, _D, _D, ctypes.c_double]),
             "gfo_momenta_dot": (ctypes.c_double, [_I, _D]),
+            "gfo_md_update_unfused": (None, [_I, _D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
             "gfo_potential": (ctypes.c_double, [_I, _D, ctypes.c_double]),
             "gfo_flow_force": (None, [_I, _D, _D]),
             "gfo_flow": (None, [_I, _D, ctypes.c_double, ctypes.c_int, ctypes.c_int, _D]),
@@ -156,6 +157,13 @@ def kick(dims, P, F, eps):
 
 def exp_update(dims, U, P, eps):
     lib().gfo_exp_update(_dims(dims), _p(U), _p(np.ascontiguousarray(P)), float(eps))
+
+
+def md_update_unfused(dims, U, P, coeff, eps_p, eps_u):
+    """update_momenta!(eps_p) then update_gaugefields!(eps_u) in the reference's pass structure,
+    in place on U and P (C-contiguous complex128 / float64)"""
+    assert U.flags["C_CONTIGUOUS"] and P.flags["C_CONTIGUOUS"]
+    lib().gfo_md_update_unfused(_dims(dims), _p(U), _p(P), float(coeff), float(eps_p), float(eps_u))
 
 
 def momenta_dot(dims, P):

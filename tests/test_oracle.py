@@ -223,6 +223,21 @@ def test_exp_update_and_kick(oracle):
     assert abs(oracle.momenta_dot(D4, P) - np.sum(P * P)) < 1e-10
 
 
+@pytest.mark.parametrize("dims", [D4, (4, 2, 3, 6)])
+def test_unfused_md_update_is_force_kick_exp(oracle, dims):
+    """the reference-structured CPU baseline (104 whole-lattice passes through temporaries) is
+    the same map as the finite-difference-pinned force, the kick and the expm-pinned update"""
+    U = hot(dims, 44)
+    P = gen.gaussian_momenta(dims, 45)
+    U1, P1 = U.copy(), P.copy()
+    oracle.md_update_unfused(dims, U1, P1, COEFF, 0.02, 0.03)
+    U2, P2 = U.copy(), P.copy()
+    oracle.kick(dims, P2, oracle.force(dims, U2, COEFF), 0.02)
+    oracle.exp_update(dims, U2, P2, 0.03)
+    assert np.abs(P1 - P2).max() < 1e-14
+    assert np.abs(U1 - U2).max() < 1e-14
+
+
 # ---------------------------------------------------------------- Wilson flow
 
 
