@@ -89,33 +89,64 @@ def time_kernel(L, fn, reps):
     return L.event_elapsed_ms(2, 3) / reps
 
 
-def cpu_baseline(dims, min_seconds=10.0, max_updates=200):
-    """The C oracle's MD update (force + kick + exp, FP64, OpenMP over the host's cores),
-    repeated until at least min_seconds of CPU work have run."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(dims, min_seconds=8.0, max_updates=200):
+    """The C oracle's MD update (force + kick + exp, FP64, OpenMP over the host's cores) in two
+    forms, each repeated until at least min_seconds of CPU work have run: the reference's call
+    structure (md_force! per direction: six staples as whole-lattice products into temporaries,
+    add_U!, mul!, Traceless_antihermitian_add!, then add_U! and per-direction exptU!/mul!/
+    substitute_U!: 104 passes per update) -- the value -- and the fused restatement (one pass
+    per kernel)."""
     import oracle.gforacle as orc
 
     gf = __graft_entry__.load_package()
     gen = gf.generators
-    U = gen.cold_links(dims)
-    orc.exp_update(dims, U, gen.algebra_coefficients(dims, 0.3, 20260), 1.0)
-    P = gen.gaussian_momenta(dims, 20261)
     V = int(np.prod(dims))
-    F = np.zeros((4, V, 8))
-    t0 = time.perf_counter()
-    updates = 0
-    while updates < max_updates and (updates < 2 or time.perf_counter() - t0 < min_seconds):
-        F = orc.force(dims, U, COEFF)
-        orc.kick(dims, P, F, 0.02)
+    omega = gen.algebra_coefficients(dims, 0.3, 20260)
+
+    def run(update):
+        U = gen.cold_links(dims)
+        orc.exp_update(dims, U, omega, 1.0)
+        P = gen.gaussian_momenta(dims, 20261)
+        t0 = time.perf_counter()
+        n = 0
+        while n < max_updates and (n < 2 or time.perf_counter() - t0 < min_seconds):
+            update(U, P)
+            n += 1
+        return n, time.perf_counter() - t0
+
+    def unfused(U, P):
+        orc.md_update_unfused(dims, U, P, COEFF, 0.02, 0.02)
+
+    def fused(U, P):
+        orc.kick(dims, P, orc.force(dims, U, COEFF), 0.02)
         orc.exp_update(dims, U, P, 0.02)
-        updates += 1
-    dt = time.perf_counter() - t0
+
+    n_ref, t_ref = run(unfused)
+    n_fus, t_fus = run(fused)
+    lat = "x".join(map(str, dims))
     return {
-        "value": 4 * V * updates / dt,
+        "value": 4 * V * n_ref / t_ref,
         "unit": "link-updates/s",
         "cores": orc.get_threads(),
         "kind": "port",
-        "sample": f"C oracle (OpenMP, FP64), {updates} MD updates (force + kick + exp) of a "
-        f"{'x'.join(map(str, dims))} lattice, {dt:.1f} s",
+        "sample": f"C oracle (OpenMP, FP64) in the reference's call structure (104 whole-lattice passes "
+        f"per MD update), {n_ref} MD updates (force + kick + exp) of a {lat} lattice, {t_ref:.1f} s",
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
+        "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+        "fused_port": {"value": 4 * V * n_fus / t_fus, "unit": "link-updates/s",
+                       "sample": f"fused restatement (one pass per kernel), {n_fus} MD updates, {t_fus:.1f} s"},
     }
 
 
@@ -259,6 +290,117 @@ def stout_hmc_thermalized(gf, dims=DIMS, ntherm=6, npairs=8):
     }
 
 
+def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232):
+    """Which rounding sets the FP32-bulk dH deviation (3-level stout HMC, rho 0.12, 2MN 16 steps,
+    tau 1, 32^4).  Thermalization: nforce FP64 trajectories accepted unconditionally from
+    omega ~ N(0, 0.3^2) links, then nmetro with Metropolis (acceptance reported).  Each of ndraws
+    momentum draws starts every variant from the same configuration (link rows and momenta as
+    float holds them) and compares its dH with the FP64 trajectory's:
+      fp32_bulk            the FP32-bulk form (FP32 links, momenta, forces, stout C; fused steps);
+      round_momenta        FP64, momenta rounded to float after every kick;
+      round_links          FP64, links rounded to float after every update (after the SU(3)
+                           projection in double, as the FP32 link updates do);
+      round_links_noproj   the same without the projection (links projected only at the start);
+      round_both           round_links and round_momenta.
+    The FP64 variants step 2MN from the host (kick, update, kick, update, kick per step) with
+    gf_round_to_float between the steps; the reference dH is the same unrounded sequence."""
+    gen = gf.generators
+    lam = 0.1931833275037836
+    L64 = gf.Lattice(dims, precision="f64")
+    L64.links_from_algebra(gen.algebra_coefficients(dims, 0.3, seed))
+    L64.stout_config(3, 0.12, "store")
+    rng = np.random.default_rng(seed + 1)
+    therm, acc = [], 0
+    for k in range(nforce + nmetro):
+        L64.save_links()
+        L64.gaussian_momenta(seed, 10 ** 6 + k)
+        dH = L64.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True)[2]
+        therm.append(dH)
+        if k >= nforce:
+            if rng.random() < np.exp(min(0.0, -dH)):
+                acc += 1
+            else:
+                L64.restore_links()
+        log(f"[precision] thermalization {k}: dH {dH:.4g} plaquette {L64.plaquette():.6f}")
+    L64.round_to_float(links=True, project=True)  # the start every form can hold
+    U0 = L64.download_links()
+    plaq0 = L64.plaquette()
+    L32 = gf.Lattice(dims, precision="f32")
+    L32.stout_config(3, 0.12, "store")
+
+    def stepped(rl, rm, project=True):
+        H0 = L64.hamiltonian(COEFF)
+        eps = 1.0 / 16
+        for _ in range(16):
+            for kind, h in (("p", lam * eps), ("u", eps / 2), ("p", (1 - 2 * lam) * eps), ("u", eps / 2),
+                            ("p", lam * eps)):
+                if kind == "p":
+                    L64.force_kick(COEFF, h)
+                    if rm:
+                        L64.round_to_float(momenta=True)
+                else:
+                    L64.exp_update(h)
+                    if rl:
+                        L64.round_to_float(links=True, project=project)
+        return L64.hamiltonian(COEFF) - H0
+
+    variants = {"round_momenta": (False, True, True), "round_links": (True, False, True),
+                "round_links_noproj": (True, False, False), "round_both": (True, True, True)}
+    dev = {k: [] for k in ["fp32_bulk", "fp64_fused"] + list(variants)}
+    ref = []
+    t0 = time.perf_counter()
+    for d in range(ndraws):
+        L64.upload_links(U0)
+        L64.gaussian_momenta(seed + 7, d)
+        L64.round_to_float(momenta=True)
+        P0 = L64.download_momenta()
+        dh64 = stepped(False, False)
+        ref.append(dh64)
+        for name, (rl, rm, pj) in variants.items():
+            L64.upload_links(U0)
+            L64.upload_momenta(P0)
+            dev[name].append(stepped(rl, rm, pj) - dh64)
+        L64.upload_links(U0)
+        L64.upload_momenta(P0)
+        dev["fp64_fused"].append(L64.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True)[2] - dh64)
+        L32.upload_links(U0)
+        L32.upload_momenta(P0)
+        dev["fp32_bulk"].append(L32.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True)[2] - dh64)
+        log(f"[precision] draw {d}: dH64 {dh64:.5g} " + " ".join(f"{k} {v[-1]:+.2e}" for k, v in dev.items()))
+    L64.close()
+    L32.close()
+
+    def stats(x):
+        x = np.asarray(x)
+        return {"rms": float(np.sqrt(np.mean(x ** 2))), "mean": float(x.mean()),
+                "mean_stderr": float(x.std(ddof=1) / np.sqrt(len(x))) if len(x) > 1 else None,
+                "median_abs": float(np.median(np.abs(x))), "max_abs": float(np.abs(x).max()),
+                "frac_within_1e-3": float(np.mean(np.abs(x) <= 1e-3))}
+
+    return {"lattice": "x".join(map(str, dims)), "levels": 3, "rho": 0.12, "integrator": "2MN", "steps": 16,
+            "tau": 1.0, "draws": ndraws, "thermalization_dH": therm,
+            "metropolis_acceptance": acc / nmetro if nmetro else None, "plaquette_start": plaq0,
+            "dH_fp64": {"rms": float(np.sqrt(np.mean(np.square(ref)))), "values": ref},
+            "deviation_from_fp64": {k: stats(v) for k, v in dev.items()},
+            "deviations": dev, "wall_s": time.perf_counter() - t0}
+
+
+def stout_force_timing(gf, dims=DIMS, nforce=20):
+    """3-level stout force (rho 0.12, stored C) at 32^4 from seeded omega ~ N(0, 0.3^2) links:
+    ms per force in FP64 and FP32 bulk (HIP events on the library stream, after 5 untimed)"""
+    gen = gf.generators
+    out = {"lattice": "x".join(map(str, dims)), "levels": 3, "rho": 0.12, "forces": nforce}
+    for prec in ("f64", "f32"):
+        L = gf.Lattice(dims, precision=prec)
+        L.links_from_algebra(gen.algebra_coefficients(dims, 0.3, 20260))
+        L.stout_config(3, 0.12, "store")
+        for _ in range(5):
+            L.stout_force(COEFF)
+        out[f"{prec}_ms_per_force"] = time_kernel(L, lambda: L.stout_force(COEFF), nforce)
+        L.close()
+    return out
+
+
 def production_stout_hmc(gf, dims=(48, 48, 48, 48), ntraj=50, check_every=10, seed=4848, ntherm=0):
     """Production-size stout HMC: SU(3) 48^4, beta 6, 3 stout levels at rho 0.12, Omelyan 2MN
     16 steps, tau 1; FP32 links, momenta and forces (FP64 action/dH sums, Metropolis, CH
@@ -331,13 +473,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-stout", action="store_true", help="skip the stout-HMC s/traj lines")
-    ap.add_argument("--stout-therm", type=int, default=6, help="FP64 thermalization trajectories")
+    ap.add_argument("--stout-therm", type=int, default=12, help="FP64 thermalization trajectories")
     ap.add_argument("--stout-draws", type=int, default=8,
                     help="momentum draws of the FP64 vs FP32-bulk dH comparison")
     ap.add_argument("--stout-only", action="store_true", help="run only the stout-HMC and flow comparisons")
     ap.add_argument("--no-next", action="store_true", help="skip the heatbath / random-field / field-strength / ILDG timings")
     ap.add_argument("--next-only", action="store_true",
                     help="run only the heatbath / random-field / field-strength / ILDG timings")
+    ap.add_argument("--stout-force", type=int, default=0, metavar="N",
+                    help="run only N timed 3-level stout forces at 32^4 in FP64 and FP32 bulk")
+    ap.add_argument("--precision-study", type=int, default=0, metavar="NDRAWS",
+                    help="run only the FP32-bulk dH rounding study (32^4 stout HMC) over NDRAWS draws")
     ap.add_argument("--production", type=int, default=0, metavar="NTRAJ",
                     help="run only the 48^4 FP32-bulk stout HMC with NTRAJ trajectories")
     ap.add_argument("--production-therm", type=int, default=0, metavar="N",
@@ -349,6 +495,12 @@ def main():
         sys.exit(2)
 
     gf = __graft_entry__.load_package()
+    if args.stout_force:
+        print(json.dumps({"stout_force": stout_force_timing(gf, nforce=args.stout_force)}))
+        return
+    if args.precision_study:
+        print(json.dumps({"precision_study": precision_study(gf, ndraws=args.precision_study)}))
+        return
     if args.production:
         print(json.dumps({"production_stout_hmc": production_stout_hmc(gf, ntraj=args.production,
                                                                        ntherm=args.production_therm)}))
@@ -395,16 +547,21 @@ def main():
     ms_force = time_kernel(L, lambda: L.force_kick(COEFF, 0.0), 10)
     ms_exp = time_kernel(L, lambda: L.exp_update(0.02), 10)
     ms_plaq = time_kernel(L, lambda: L.plaquette_sum(), 10)
-    # device-to-device copy of the link field (hipMemcpyAsync): the measured stream-copy
-    # rate, and the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters
-    ms_copy = time_kernel(L, L.save_links, 10)
+    # the library's streaming copy of the link field (k_stream_copy, 16-byte non-temporal loads
+    # and stores): the measured HBM ceiling, and the calibration kernel of the FETCH_SIZE /
+    # WRITE_SIZE counters
     copy_bytes = 2 * 48 * 8 * V  # read + write of one link field (12 reals per link)
-    copy_gbps = copy_bytes / (ms_copy * 1e-3) / 1e9
+    copy_gbps = L.stream_copy_rate(20) / 1e9
+    ms_copy = copy_bytes / (copy_gbps * 1e9) * 1e3
     plaq = L.plaquette()
     L.close()
     def kern(name, ms, key="ms"):
+        # GBps and frac: the bytes this device layout moves (12-real links), the HBM utilization
+        # proper; reference_GBps: the 18-real algorithmic basis of the path
+        gbps = B[name] * V / (ms * 1e-3) / 1e9
         return {key: ms, "bytes_per_site": B[name], "reference_bytes_per_site": REF_BYTES[name],
-                "GBps": B[name] * V / (ms * 1e-3) / 1e9}
+                "GBps": gbps, "frac": gbps / HBM_PEAK_GBS, "frac_of_measured_copy": gbps / copy_gbps,
+                "reference_GBps": REF_BYTES[name] * V / (ms * 1e-3) / 1e9}
 
     kernels = {
         "md_update_fused": kern("md_update_fused", ms_per_step),
@@ -412,7 +569,7 @@ def main():
         "exp_update": kern("exp_update", ms_exp),
         "plaquette": kern("plaquette", ms_plaq, "ms_incl_readback"),
     }
-    kernels["link_copy"] = {"ms": ms_copy, "bytes": copy_bytes, "GBps": copy_gbps}
+    kernels["stream_copy"] = {"ms": ms_copy, "bytes": copy_bytes, "GBps": copy_gbps, "frac": copy_gbps / HBM_PEAK_GBS}
     log(f"[bench] kernels {json.dumps(kernels)}")
 
     stout = None

@@ -15,6 +15,7 @@ GF_QPQ, GF_PQP, GF_OMELYAN2MN, GF_FUSED = 0, 1, 2, 0x100
 GF_STOUT_STORE, GF_STOUT_RECOMPUTE, GF_STOUT_AUTO = 0, 1, 2
 GF_TOPO_PLAQUETTE, GF_TOPO_CLOVER, GF_TOPO_IMPROVED, GF_TOPO_RECTANGLE = 0, 1, 2, 3
 GF_FLOW_NOBS = 7
+GF_ROUND_LINKS, GF_ROUND_PROJECT, GF_ROUND_MOMENTA = 1, 2, 4
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -71,6 +72,8 @@ SIGNATURES = {
     "gf_ildg_read": (ctypes.c_int, [ctypes.c_char_p, _I, _D, _I]),
     "gf_ildg_write": (ctypes.c_int, [ctypes.c_char_p, _I, _D, ctypes.c_int]),
     "gf_ildg_last_error": (ctypes.c_char_p, []),
+    "gf_round_to_float": (ctypes.c_int, [_CTX, ctypes.c_int]),
+    "gf_stream_copy": (ctypes.c_int, [_CTX, ctypes.c_int, _D]),
     "gf_event_record": (ctypes.c_int, [_CTX, ctypes.c_int]),
     "gf_event_elapsed": (ctypes.c_int, [_CTX, ctypes.c_int, ctypes.c_int, _F]),
 }

@@ -990,6 +990,31 @@ int gf_stout_force_kick(gf_ctx* c, double coeff, double eps) {
     return kick(c, coeff, eps);
 }
 
+int gf_round_to_float(gf_ctx* c, int fields) {
+    CTXCHK(c);
+    if (c->prec != GF_F64) return fail(c, GF_ESTATE, "rounding to float applies to an FP64 context");
+    if (fields & ~(GF_ROUND_LINKS | GF_ROUND_PROJECT | GF_ROUND_MOMENTA))
+        return fail(c, GF_EINVAL, "unknown rounding flags 0x%x", fields);
+    HIPCHK(c, launch_round_f32((double*)c->U, (double*)c->P, c->V, (fields & GF_ROUND_LINKS) != 0,
+                               (fields & GF_ROUND_PROJECT) != 0, (fields & GF_ROUND_MOMENTA) != 0, c->stream));
+    return GF_OK;
+}
+
+int gf_stream_copy(gf_ctx* c, int reps, double* bytes_per_s) {
+    CTXCHK(c);
+    if (reps <= 0 || !bytes_per_s) return fail(c, GF_EINVAL, "reps must be positive and bytes_per_s non-null");
+    const size_t n = link_bytes(c);
+    HIPCHK(c, launch_stream_copy(c->U, c->Ualt, n, c->stream));  // untimed first pass
+    HIPCHK(c, hipEventRecord(c->ev[GF_NEVENTS - 2], c->stream));
+    for (int r = 0; r < reps; r++) HIPCHK(c, launch_stream_copy(c->U, c->Ualt, n, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[GF_NEVENTS - 1], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[GF_NEVENTS - 1]));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[GF_NEVENTS - 2], c->ev[GF_NEVENTS - 1]));
+    *bytes_per_s = 2.0 * (double)n * reps / (1e-3 * (double)ms);
+    return GF_OK;
+}
+
 int gf_event_record(gf_ctx* c, int slot) {
     CTXCHK(c);
     if (slot < 0 || slot >= GF_NEVENTS) return fail(c, GF_EINVAL, "event slot %d out of range", slot);

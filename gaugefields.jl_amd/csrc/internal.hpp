@@ -23,6 +23,10 @@ inline int grid_sites_mu(int V) { return (V + 63) / 64; }
 enum ForceMode { FORCE_STORE = 0, FORCE_KICK = 1, FORCE_KICK_EXP = 2 };
 
 // ---- k_gauge.hip ----
+// dst = src, bytes a multiple of 16 (streaming copy: the measured HBM ceiling)
+hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
+// FP64 link rows 0-1 (optionally SU(3)-projected first) and/or momenta rounded to float in place
+hipError_t launch_round_f32(double* U, double* P, int V, bool links, bool project, bool momenta, hipStream_t st);
 // out[0] = sum of partial[0..n) (one workgroup, fixed order)
 hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStream_t st);
 template <typename T>

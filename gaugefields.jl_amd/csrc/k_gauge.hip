@@ -4,6 +4,8 @@
 // Stencil kernels run one thread per (site, link direction) (site_mu_index): the four
 // directions of a group of 64 sites are the four wavefronts of one workgroup, and groups are
 // visited in 4D Morton order per XCD, so neighbour links are re-read from L1/L2 instead of HBM.
+#include <algorithm>
+
 #include "internal.hpp"
 
 namespace gf {
@@ -255,10 +257,72 @@ __global__ __launch_bounds__(kBlock) void k_axpby(T* __restrict__ y, const T* __
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = a * x[i] + b * y[i];
 }
+// The FP32-bulk storage rounding applied to FP64 fields (precision study): rows 0 and 1 of every
+// link, after the double-precision SU(3) projection the FP32 link updates apply when PROJECT,
+// rounded to the nearest float in place; momenta: every coefficient.
+template <bool PROJECT>
+__global__ __launch_bounds__(kBlock) void k_round_links_f32(double* __restrict__ U, int V) {
+    const int s = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (s >= V) return;
+    for (int mu = 0; mu < 4; mu++) {
+        M3<double> m = ldu(U, (unsigned)V, mu, (unsigned)s);
+        if (PROJECT) reunitarize_f64(m);
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            m.m[k].r = (double)(float)m.m[k].r;
+            m.m[k].i = (double)(float)m.m[k].i;
+        }
+        stu(U, (unsigned)V, mu, (unsigned)s, m);
+    }
+}
+__global__ __launch_bounds__(kBlock) void k_round_f32(double* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = (double)(float)p[i];
+}
+
+// Streaming copy of n 16-byte words (the measured HBM ceiling): each thread moves four words
+// per pass of a grid-stride loop, loads and stores non-temporal.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
+                                                        size_t n) {
+    const size_t stride = (size_t)gridDim.x * kBlock * 4;
+    for (size_t i = (size_t)blockIdx.x * kBlock * 4 + threadIdx.x; i < n; i += stride) {
+        u32x4 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (i + (size_t)k * kBlock < n) v[k] = __builtin_nontemporal_load(src + i + (size_t)k * kBlock);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (i + (size_t)k * kBlock < n) __builtin_nontemporal_store(v[k], dst + i + (size_t)k * kBlock);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t st) {
+    const size_t n = bytes / 16;
+    if (n == 0) return hipSuccess;
+    const size_t per_block = (size_t)kBlock * 4;
+    const unsigned nb = (unsigned)std::min<size_t>((n + per_block - 1) / per_block, 256 * 16);
+    k_stream_copy<<<nb, kBlock, 0, st>>>((const u32x4*)src, (u32x4*)dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_round_f32(double* U, double* P, int V, bool links, bool project, bool momenta, hipStream_t st) {
+    if (links) {
+        if (project)
+            k_round_links_f32<true><<<grid_for(V), kBlock, 0, st>>>(U, V);
+        else
+            k_round_links_f32<false><<<grid_for(V), kBlock, 0, st>>>(U, V);
+    }
+    if (momenta) {
+        const size_t n = (size_t)32 * V;
+        k_round_f32<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(P, n);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStream_t st) {
     k_sum_partials<<<1, 1024, 0, st>>>(partial, n, out);
     return hipGetLastError();

@@ -283,3 +283,18 @@ class Lattice:
         ms = ctypes.c_float()
         self._c(self._L.gf_event_elapsed(self._ctx, int(a), int(b), ctypes.byref(ms)))
         return ms.value
+
+    def stream_copy_rate(self, reps=20):
+        """bytes/s (read + written) of the library's streaming copy of the link field"""
+        r = ctypes.c_double()
+        self._c(self._L.gf_stream_copy(self._ctx, int(reps), ctypes.byref(r)))
+        return r.value
+
+    def round_to_float(self, links=False, momenta=False, project=True):
+        """FP64 context: round link rows 0-1 (SU(3)-projected in double first if project) and/or
+        momenta to the nearest float in place (the FP32-bulk storage rounding)"""
+        from ._lib import GF_ROUND_LINKS, GF_ROUND_MOMENTA, GF_ROUND_PROJECT
+
+        f = (GF_ROUND_LINKS if links else 0) | (GF_ROUND_MOMENTA if momenta else 0)
+        f |= GF_ROUND_PROJECT if (links and project) else 0
+        self._c(self._L.gf_round_to_float(self._ctx, f))

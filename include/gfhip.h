@@ -206,7 +206,22 @@ int gf_ildg_write(const char *path, const int *dims, const double *ref, int prec
 /* message of the last failing gf_ildg_* call on this thread */
 const char *gf_ildg_last_error(void);
 
+/* ---- precision study ---------------------------------------------------------------- */
+/* No counterpart in the reference: the storage rounding of the FP32-bulk form applied to an FP64
+ * context, in place, so that an FP64 trajectory stepped from the host can carry exactly one of the
+ * FP32 form's roundings.  GF_ROUND_LINKS: rows 0 and 1 of every link rounded to the nearest float
+ * (with GF_ROUND_PROJECT: after the SU(3) projection in double that the FP32 link updates apply);
+ * GF_ROUND_MOMENTA: every momentum coefficient.  GF_ESTATE on an FP32 context. */
+enum { GF_ROUND_LINKS = 1, GF_ROUND_PROJECT = 2, GF_ROUND_MOMENTA = 4 };
+int gf_round_to_float(gf_ctx *ctx, int fields);
+
 /* ---- timing (HIP events on the context's stream) ----------------------------------- */
+/* No counterpart in the reference: the device's streaming-copy rate, the measured HBM ceiling
+ * quoted beside the 8 TB/s peak.  Copies the context's link field into its second link buffer
+ * (the current links are unchanged) reps times after one untimed pass, timed with events
+ * GF_NEVENTS-2 and GF_NEVENTS-1 (slots 14, 15) on the context's stream; *bytes_per_s = read +
+ * written bytes per second. */
+int gf_stream_copy(gf_ctx *ctx, int reps, double *bytes_per_s);
 int gf_event_record(gf_ctx *ctx, int slot); /* slot 0..15 */
 int gf_event_elapsed(gf_ctx *ctx, int slot_a, int slot_b, float *ms);
 

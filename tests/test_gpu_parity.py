@@ -405,3 +405,41 @@ def test_unaligned_volume_stout_flow_md(Lat, oracle, mode):
     Eref = oracle.flow(dims, U2, 0.02, 4, 2)
     assert np.all(np.abs(E - Eref) < 1e-12 * np.abs(Eref))
     assert maxdiff(L.download_links(), U2) < 1e-13
+
+
+# ------------------------------------------------------------------ precision-study rounding
+
+
+def _gram_schmidt_rows(M):
+    """rows 0 and 1 of each 3x3 (..., 3, 3) orthonormalized in double, row 2 = conj(r0 x r1)"""
+    r0 = M[..., 0, :] / np.linalg.norm(M[..., 0, :], axis=-1, keepdims=True)
+    r1 = M[..., 1, :] - np.sum(np.conj(r0) * M[..., 1, :], axis=-1, keepdims=True) * r0
+    r1 = r1 / np.linalg.norm(r1, axis=-1, keepdims=True)
+    return np.stack([r0, r1, np.conj(np.cross(r0, r1))], axis=-2)
+
+
+def test_round_to_float_is_the_fp32_storage_rounding(Lat, oracle, gf):
+    """gf_round_to_float on an FP64 context: momenta equal numpy's float32 rounding; links equal
+    rows 0-1 of the double Gram-Schmidt projection rounded to float, with or without the
+    projection; an FP32 context refuses it"""
+    dims = (4, 4, 4, 6)
+    U = alg_links(oracle, dims, 0.3, 61)
+    P = gen.gaussian_momenta(dims, 62)
+    L = Lat(dims)
+    L.upload_links(U)
+    L.upload_momenta(P)
+    Ud = gen.links_to_matrices(L.download_links())
+    L.round_to_float(momenta=True)
+    assert np.array_equal(L.download_momenta(), P.astype(np.float32).astype(np.float64))
+    assert np.array_equal(gen.links_to_matrices(L.download_links()), Ud)  # links untouched
+    for project in (False, True):
+        L.upload_links(U)
+        L.round_to_float(links=True, project=project)
+        got = gen.links_to_matrices(L.download_links())
+        M = _gram_schmidt_rows(Ud) if project else Ud.copy()
+        M[..., :2, :] = M[..., :2, :].astype(np.complex64).astype(np.complex128)
+        M[..., 2, :] = np.conj(np.cross(M[..., 0, :], M[..., 1, :]))
+        assert np.abs(got - M).max() < 1e-15, project
+    L32 = Lat(dims, "f32")
+    with pytest.raises(gf.GFError):
+        L32.round_to_float(momenta=True)
