@@ -613,11 +613,72 @@ __device__ __forceinline__ void ch_coeffs(double c0, double c1, dc f[3], dc b1[3
     }
 }
 
-// c1 below which exp(iH) = 1 + iH - H^2/2 (error < 1e-30)
-#define GF_CH_EXP_TINY 1e-20
-// c1 below which C star dexp(Q)/dQ is summed from its power series
-#define GF_CH_DER_SERIES 1e-3
-#define GF_CH_DER_TERMS 9
+// The same coefficients from the exponential series reduced by Cayley-Hamilton: with
+// H^3 = c1 H + c0 (traceless H), H^n = a_n + b_n H + c_n H^2 where (a, b, c)_0 = (1, 0, 0) and
+// a_{n+1} = c0 c_n, b_{n+1} = a_n + c1 c_n, c_{n+1} = b_n, so f_j = sum_n i^n / n! (a_n, b_n, c_n)_j;
+// b_1j = df_j/dc1 and b_2j = df_j/dc0 follow by differentiating the recursion.  For eigenvalues
+// of modulus at most 1 (c1 <= 3/4, since max lambda^2 <= 4 c1 / 3) the terms after N are below
+// e / N! in norm: N = 18 for f (1.7e-16), 20 with the derivatives, whose terms carry a factor n
+// more (checked against scipy's expm and expm_frechet: 4e-16 and 1.4e-15).  Straight-line FMAs
+// instead of acos, three sincos, square roots and divisions; no special case at small c1.
+#define GF_CH_SERIES_C1 0.75
+template <bool WANT_B>
+__device__ __forceinline__ void ch_coeffs_series(double c0, double c1, dc f[3], dc b1[3], dc b2[3]) {
+    constexpr int N = WANT_B ? 20 : 18;
+    double a = 1.0, b = 0.0, c = 0.0;
+    double a0 = 0.0, bb0 = 0.0, cc0 = 0.0, a1 = 0.0, bb1 = 0.0, cc1 = 0.0;  // d/dc0, d/dc1
+    double fr[3] = {0.0, 0.0, 0.0}, fi[3] = {0.0, 0.0, 0.0};
+    double g1r[3] = {0.0, 0.0, 0.0}, g1i[3] = {0.0, 0.0, 0.0}, g2r[3] = {0.0, 0.0, 0.0}, g2i[3] = {0.0, 0.0, 0.0};
+    double w = 1.0;  // 1/n!
+#pragma unroll
+    for (int n = 0; n < N; n++) {
+        const double s = ((n >> 1) & 1) ? -w : w;  // i^n / n! = s (n even) or i s (n odd)
+        double* pf = (n & 1) ? fi : fr;
+        pf[0] = fma(s, a, pf[0]);
+        pf[1] = fma(s, b, pf[1]);
+        pf[2] = fma(s, c, pf[2]);
+        if (WANT_B) {
+            double* p1 = (n & 1) ? g1i : g1r;
+            double* p2 = (n & 1) ? g2i : g2r;
+            p1[0] = fma(s, a1, p1[0]);
+            p1[1] = fma(s, bb1, p1[1]);
+            p1[2] = fma(s, cc1, p1[2]);
+            p2[0] = fma(s, a0, p2[0]);
+            p2[1] = fma(s, bb0, p2[1]);
+            p2[2] = fma(s, cc0, p2[2]);
+            const double na0 = fma(c0, cc0, c), na1 = c0 * cc1;
+            const double nb0 = fma(c1, cc0, a0), nb1 = fma(c1, cc1, a1 + c);
+            cc0 = bb0;
+            cc1 = bb1;
+            a0 = na0;
+            a1 = na1;
+            bb0 = nb0;
+            bb1 = nb1;
+        }
+        const double na = c0 * c, nb = fma(c1, c, a);
+        c = b;
+        a = na;
+        b = nb;
+        w *= 1.0 / (double)(n + 1);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        f[j] = {fr[j], fi[j]};
+        if (WANT_B) {
+            b1[j] = {g1r[j], g1i[j]};
+            b2[j] = {g2r[j], g2i[j]};
+        }
+    }
+}
+// f_j (and b_1j, b_2j) of exp(iH): the series for c1 <= 3/4, the closed form beyond
+template <bool WANT_B>
+__device__ __forceinline__ void exp_coeffs(double c0, double c1, dc f[3], dc b1[3], dc b2[3]) {
+    if (c1 <= GF_CH_SERIES_C1)
+        ch_coeffs_series<WANT_B>(c0, c1, f, b1, b2);
+    else
+        ch_coeffs<WANT_B>(c0, c1, f, b1, b2);
+}
+
 
 // exp(iH) for Hermitian traceless H
 template <typename T>
@@ -626,19 +687,8 @@ __device__ __forceinline__ M3<T> expi_herm(const M3<T>& H) {
     herm_invariants(H, c0, c1);
     M3<T> H2 = herm_sq(H);
     M3<T> E;
-    if (c1 < GF_CH_EXP_TINY) {
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-            E.m[k].r = -H.m[k].i - T(0.5) * H2.m[k].r;
-            E.m[k].i = H.m[k].r - T(0.5) * H2.m[k].i;
-        }
-        E.m[0].r += T(1);
-        E.m[4].r += T(1);
-        E.m[8].r += T(1);
-        return E;
-    }
     dc f[3];
-    ch_coeffs<false>(c0, c1, f, nullptr, nullptr);
+    exp_coeffs<false>(c0, c1, f, nullptr, nullptr);
     const T f0r = T(f[0].r), f0i = T(f[0].i), f1r = T(f[1].r), f1i = T(f[1].i);
     const T f2r = T(f[2].r), f2i = T(f[2].i);
 #pragma unroll
@@ -682,28 +732,8 @@ __device__ __forceinline__ M3<T> cdexpqdq(const M3<T>& C, const M3<T>& Q) {
     }
     double c0, c1;
     herm_invariants(H, c0, c1);
-    if (c1 < GF_CH_DER_SERIES) {
-        // L = sum_{n>=1} 1/n! sum_{k<n} Q^(n-1-k) C Q^k ;  Ln+1 = Q Ln + C Q^n
-        M3<T> L = C, Ln = C, CQn = C;
-        T fact = T(1);
-#pragma unroll 1
-        for (int n = 2; n <= GF_CH_DER_TERMS; n++) {
-            CQn = mul(CQn, Q);
-            M3<T> t = mul(Q, Ln);
-            acc(t, CQn);
-            Ln = t;
-            fact *= T(n);
-            T inv = T(1) / fact;
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                L.m[k].r = fma(inv, Ln.m[k].r, L.m[k].r);
-                L.m[k].i = fma(inv, Ln.m[k].i, L.m[k].i);
-            }
-        }
-        return L;
-    }
     dc f[3], b1[3], b2[3];
-    ch_coeffs<true>(c0, c1, f, b1, b2);
+    exp_coeffs<true>(c0, c1, f, b1, b2);
     M3<T> H2 = herm_sq(H);
     // traces tr(C), tr(C H), tr(C H^2) in double
     dc t0 = {0.0, 0.0}, t1 = {0.0, 0.0}, t2 = {0.0, 0.0};
@@ -755,55 +785,8 @@ __device__ __forceinline__ void exp_dexp(const M3<T>& Q, const M3<T>& X, M3<T>& 
     double c0, c1;
     herm_invariants(H, c0, c1);
     const M3<T> H2 = herm_sq(H);
-    if (c1 < GF_CH_DER_SERIES) {
-        if (c1 < GF_CH_EXP_TINY) {
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                E.m[k].r = -H.m[k].i - T(0.5) * H2.m[k].r;
-                E.m[k].i = H.m[k].r - T(0.5) * H2.m[k].i;
-            }
-            E.m[0].r += T(1);
-            E.m[4].r += T(1);
-            E.m[8].r += T(1);
-        } else {
-            dc f[3];
-            ch_coeffs<false>(c0, c1, f, nullptr, nullptr);
-            const T f0r = T(f[0].r), f0i = T(f[0].i), f1r = T(f[1].r), f1i = T(f[1].i);
-            const T f2r = T(f[2].r), f2i = T(f[2].i);
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                E.m[k].r = f1r * H.m[k].r - f1i * H.m[k].i + f2r * H2.m[k].r - f2i * H2.m[k].i;
-                E.m[k].i = f1r * H.m[k].i + f1i * H.m[k].r + f2r * H2.m[k].i + f2i * H2.m[k].r;
-            }
-            E.m[0].r += f0r;
-            E.m[0].i += f0i;
-            E.m[4].r += f0r;
-            E.m[4].i += f0i;
-            E.m[8].r += f0r;
-            E.m[8].i += f0i;
-        }
-        // L = sum_{n>=1} 1/n! sum_{k<n} Q^(n-1-k) X Q^k
-        M3<T> Ln = X, XQn = X;
-        L = X;
-        T fact = T(1);
-#pragma unroll 1
-        for (int n = 2; n <= GF_CH_DER_TERMS; n++) {
-            XQn = mul(XQn, Q);
-            M3<T> t = mul(Q, Ln);
-            acc(t, XQn);
-            Ln = t;
-            fact *= T(n);
-            const T inv = T(1) / fact;
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                L.m[k].r = fma(inv, Ln.m[k].r, L.m[k].r);
-                L.m[k].i = fma(inv, Ln.m[k].i, L.m[k].i);
-            }
-        }
-        return;
-    }
     dc f[3], b1[3], b2[3];
-    ch_coeffs<true>(c0, c1, f, b1, b2);
+    exp_coeffs<true>(c0, c1, f, b1, b2);
     const T f0r = T(f[0].r), f0i = T(f[0].i), f1r = T(f[1].r), f1i = T(f[1].i);
     const T f2r = T(f[2].r), f2i = T(f[2].i);
     dc t0 = {0.0, 0.0}, t1 = {0.0, 0.0}, t2 = {0.0, 0.0};
