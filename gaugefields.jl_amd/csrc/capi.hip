@@ -43,6 +43,7 @@ struct gf_ctx {
     void* D = nullptr;
     void* D2 = nullptr;
     void* G = nullptr;
+    void* G2 = nullptr;  // second Lam buffer (stored-C mode: levels chained across kernels)
     // FP32 bulk: FP64 copies of the links (and smeared levels, ping-pong) for the Hamiltonian
     void* W64[2] = {nullptr, nullptr};
     // scratch for layout conversion and reductions
@@ -271,6 +272,7 @@ int stout_alloc(gf_ctx* c) {
     TRY(alloc(c, &c->D, mat_bytes(c)));
     TRY(alloc(c, &c->D2, mat_bytes(c)));
     TRY(alloc(c, &c->G, mom_bytes(c)));  // Lam coefficients of the stout backward
+    if (c->smode == GF_STOUT_STORE) TRY(alloc(c, &c->G2, mom_bytes(c)));
     return GF_OK;
 }
 const void* level_in(gf_ctx* c, int l) { return l == 0 ? c->U : c->lev[l - 1]; }
@@ -288,33 +290,51 @@ int stout_forward_all(gf_ctx* c) {
 
 // D (dS/dU of the top level) -> D holds dS/dU_bare afterwards; or, with fmode FORCE_STORE /
 // FORCE_KICK, the last (thin-link) level forms the force from dS/dU_bare instead of storing it:
-// F = -(1/3) TA(U dS/dU_bare) into FP, or FP += eps F
-int stout_backward_all(gf_ctx* c, int fmode = -1, void* FP = nullptr, double eps = 0.0, double eps_u = 0.0) {
+// F = -(1/3) TA(U dS/dU_bare) into FP, or FP += eps F.  from_dsdu: D is not given, dS/dU of the top
+// level for the plaquette action with coefficient coeff is formed first.  With stored C every level
+// boundary is one kernel (the stencil half of level l chained with the site-local half of level
+// l - 1), and dS/dU is chained into the top level's site-local half.
+const void* top_links(gf_ctx* c) { return c->nlev > 0 ? c->lev[c->nlev - 1] : c->U; }
+
+int stout_backward_all(gf_ctx* c, int fmode = -1, void* FP = nullptr, double eps = 0.0, double eps_u = 0.0,
+                       bool from_dsdu = false, double coeff = 0.0) {
     return dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
-        for (int l = c->nlev - 1; l >= 0; l--) {
-            HIPCHK(c, launch_stout_backward<T>(c->geo, (const T*)level_in(c, l),
-                                               c->smode == GF_STOUT_STORE ? (const T*)c->Cst[l] : nullptr,
-                                               (const T*)c->D, (T*)c->D2, (T*)c->G, c->rho, c->stream,
-                                               l == 0 ? fmode : -1, (T*)FP, eps, (T*)c->Ualt, eps_u));
-            std::swap(c->D, c->D2);
+        const int top = c->nlev - 1;
+        if (c->smode != GF_STOUT_STORE) {
+            if (from_dsdu) HIPCHK(c, launch_dsdu<T>(c->geo, (const T*)top_links(c), coeff, (T*)c->D, c->stream));
+            for (int l = top; l >= 0; l--) {
+                HIPCHK(c, launch_stout_backward<T>(c->geo, (const T*)level_in(c, l), nullptr, (const T*)c->D, (T*)c->D2,
+                                                   (T*)c->G, c->rho, c->stream, l == 0 ? fmode : -1, (T*)FP, eps,
+                                                   (T*)c->Ualt, eps_u));
+                std::swap(c->D, c->D2);
+            }
+            return GF_OK;
         }
+        T* const lam[2] = {(T*)c->G, (T*)c->G2};
+        if (from_dsdu)
+            HIPCHK(c, launch_dsdu_chain<T>(c->geo, (const T*)top_links(c), coeff, (const T*)level_in(c, top),
+                                           (const T*)c->Cst[top], (T*)c->D2, lam[top & 1], c->stream));
+        else
+            HIPCHK(c, launch_stout_back_local<T>(c->geo, (const T*)level_in(c, top), (const T*)c->Cst[top],
+                                                 (const T*)c->D, (T*)c->D2, lam[top & 1], c->rho, c->stream));
+        for (int l = top; l >= 1; l--)
+            HIPCHK(c, launch_stout_back_chain<T>(c->geo, (const T*)level_in(c, l), lam[l & 1], (T*)c->D2, c->rho,
+                                                 (const T*)level_in(c, l - 1), (const T*)c->Cst[l - 1],
+                                                 lam[(l - 1) & 1], c->stream));
+        HIPCHK(c, launch_stout_back_stencil<T>(c->geo, (const T*)level_in(c, 0), lam[0], (T*)c->D2, c->rho, c->stream,
+                                               fmode, (T*)FP, eps, (T*)c->Ualt, eps_u));
+        std::swap(c->D, c->D2);
         return GF_OK;
     });
 }
-
-const void* top_links(gf_ctx* c) { return c->nlev > 0 ? c->lev[c->nlev - 1] : c->U; }
 
 // mode FORCE_STORE -> F, FORCE_KICK -> P += eps F, FORCE_KICK_EXP -> that and Ualt = exp(eps_u P) U
 // (the caller swaps the link buffers)
 int stout_force_impl(gf_ctx* c, double coeff, int mode, double eps, double eps_u = 0.0) {
     TRY(stout_forward_all(c));
-    return dispatch(c, [&](auto* tag) -> int {
-        using T = std::remove_pointer_t<decltype(tag)>;
-        HIPCHK(c, launch_dsdu<T>(c->geo, (const T*)top_links(c), coeff, (T*)c->D, c->stream));
-        // the thin-link level's stencil kernel forms the force from dS/dU_bare
-        return stout_backward_all(c, mode, mode == FORCE_STORE ? (void*)c->F : (void*)c->P, eps, eps_u);
-    });
+    // the thin-link level's stencil kernel forms the force from dS/dU_bare
+    return stout_backward_all(c, mode, mode == FORCE_STORE ? (void*)c->F : (void*)c->P, eps, eps_u, true, coeff);
 }
 
 int kick(gf_ctx* c, double coeff, double eps) {
@@ -466,7 +486,7 @@ int gf_destroy(gf_ctx* c) {
     if (!c) return GF_EINVAL;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void** bufs[] = {&c->U, &c->Ualt, &c->Ubak, &c->P, &c->F, &c->A, &c->D, &c->D2, &c->G, &c->W64[0], &c->W64[1]};
+    void** bufs[] = {&c->U, &c->Ualt, &c->Ubak, &c->P, &c->F, &c->A, &c->D, &c->D2, &c->G, &c->G2, &c->W64[0], &c->W64[1]};
     for (void** b : bufs)
         if (*b) (void)hipFree(*b);
     for (int l = 0; l < GF_MAX_LEV; l++) {
@@ -942,6 +962,7 @@ int gf_stout_config(gf_ctx* c, int nlev, double rho, int mode) {
             if (!c->lev[l]) need_levels += link_bytes(c);
             if (!c->Cst[l]) need_c += mat_bytes(c);
         }
+        need_c += c->G2 ? 0 : mom_bytes(c);
         need_levels += (c->D ? 0 : mat_bytes(c)) + (c->D2 ? 0 : mat_bytes(c)) + (c->G ? 0 : mom_bytes(c));
         mode = (double)(need_levels + need_c) * 1.1 <= (double)free_b ? GF_STOUT_STORE : GF_STOUT_RECOMPUTE;
     }

@@ -99,5 +99,23 @@ template <typename T>
 hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc,
                                  double rho, hipStream_t st, int fmode = -1, T* FP = nullptr, double eps = 0.0,
                                  T* Unew = nullptr, double eps_u = 0.0);
+// the site-local half of the stout backward alone (Cst null: C recomputed)
+template <typename T>
+hipError_t launch_stout_back_local(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc, double rho,
+                                   hipStream_t st);
+// the stencil half of the stout backward alone (fmode as above)
+template <typename T>
+hipError_t launch_stout_back_stencil(const Geo& g, const T* Uin, const T* Lc, T* Din, double rho, hipStream_t st,
+                                     int fmode, T* FP, double eps, T* Unew, double eps_u);
+// stored-C mode: the stencil half of level l (Uin, Lc; Din in: its site-local part) chained with
+// the site-local half of level l-1 (Uprev, Cprev): Din out = level l-1's site-local part, Lout =
+// Lam_{l-1} (a buffer other than Lc)
+template <typename T>
+hipError_t launch_stout_back_chain(const Geo& g, const T* Uin, const T* Lc, T* Din, double rho, const T* Uprev,
+                                   const T* Cprev, T* Lout, hipStream_t st);
+// dS/dU of the top level (Utop, coeff) chained with the top level's site-local half (Uin, Cst)
+template <typename T>
+hipError_t launch_dsdu_chain(const Geo& g, const T* Utop, double coeff, const T* Uin, const T* Cst, T* Din, T* Lout,
+                             hipStream_t st);
 
 }  // namespace gf

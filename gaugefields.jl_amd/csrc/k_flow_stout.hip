@@ -296,28 +296,15 @@ __device__ __forceinline__ void back_plane(const Geo& g, const T* __restrict__ U
     }
 }
 
-// ST (stencil_form): ST_OWN every link and Lam from memory; ST_SHARED U(y) and Lam(y) of the
+// The staple part of K6b for output direction a at site y (all planes b != a), with the
+// stencil form ST: ST_OWN every link and Lam from memory; ST_SHARED U(y) and Lam(y) of the
 // workgroup's sites through LDS (per wave 6 link and 3 Lam loads fewer); ST_XROW also the
-// x-neighbours (6 + 3 more).  At most 168 VGPRs (FP64), so that three waves fit per SIMD.
-// OUT < 0: Din += rho (staple part).  OUT = FORCE_STORE / FORCE_KICK / FORCE_KICK_EXP: the thin-link
-// level of a stout force (test/HMCstout_test_nowing.jl:99-118): Din is not stored but turned into
-// the force FP = -(1/3) TA(U_a(y) Din), or FP += eps of that, in the same kernel; FORCE_KICK_EXP
-// then also writes Unew = exp(eps_u FP) U_a(y) (update_gaugefields!, molecular_dynamics.jl:513-531)
-// to a buffer other than U, which the other threads still read.
-template <typename T, int ST, int OUT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_stout_back_stencil(
-    Geo g, const T* __restrict__ U, const T* __restrict__ Lc, T* __restrict__ Din, double rho,
-    T* __restrict__ FP, double eps, T* __restrict__ Unew, double eps_u) {
-    __shared__ T2<T> ul[ST ? 4 : 1][6][64];
-    __shared__ T2<T> hl[ST ? 4 : 1][4][64];
-    const SiteMu sm = site_mu_index(g);
-    const int a = sm.mu;
-    int y;
-    if (!stencil_site<(ST > 0)>(g, sm, y)) return;
-    int x[4];
-    decode(g, y, x);
+// x-neighbours (6 + 3 more).  The LDS forms fill ul/hl and pass one workgroup barrier.
+template <typename T, int ST>
+__device__ __forceinline__ M3<T> back_stencil_acc(const Geo& g, const T* __restrict__ U, const T* __restrict__ Lc,
+                                                  int y, const int x[4], int a, int lane, XShare<T> ul,
+                                                  PairShare<T> hl) {
     const unsigned V = g.V;
-    const int lane = (int)(threadIdx.x & 63u);
     if constexpr (ST > 0) {
         const R6<T> u = ldr(U, V, a, (unsigned)y);
         T c[8];
@@ -356,12 +343,42 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             back_plane<T, ST == ST_SHARED, false, false>(g, U, Lc, V, y, x, a, b, ypa, acc, ul, hl, lane, lxm, lxp);
         }
     }
+    return acc;
+}
+// Din_a(y) + rho * (staple part): the derivative with respect to the level's input links
+template <typename T>
+__device__ __forceinline__ M3<T> back_stencil_din(const T* __restrict__ Din, unsigned V, int a, int y, double rho,
+                                                  const M3<T>& acc) {
     M3<T> dn = ldm<T, GF_LD_NT>(Din, V, a, y);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         dn.m[k].r = fma1(rho, acc.m[k].r, dn.m[k].r);
         dn.m[k].i = fma1(rho, acc.m[k].i, dn.m[k].i);
     }
+    return dn;
+}
+
+// K6b.  At most 168 VGPRs (FP64), so that three waves fit per SIMD.
+// OUT < 0: Din += rho (staple part).  OUT = FORCE_STORE / FORCE_KICK / FORCE_KICK_EXP: the thin-link
+// level of a stout force (test/HMCstout_test_nowing.jl:99-118): Din is not stored but turned into
+// the force FP = -(1/3) TA(U_a(y) Din), or FP += eps of that, in the same kernel; FORCE_KICK_EXP
+// then also writes Unew = exp(eps_u FP) U_a(y) (update_gaugefields!, molecular_dynamics.jl:513-531)
+// to a buffer other than U, which the other threads still read.
+template <typename T, int ST, int OUT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_stout_back_stencil(
+    Geo g, const T* __restrict__ U, const T* __restrict__ Lc, T* __restrict__ Din, double rho,
+    T* __restrict__ FP, double eps, T* __restrict__ Unew, double eps_u) {
+    __shared__ T2<T> ul[ST ? 4 : 1][6][64];
+    __shared__ T2<T> hl[ST ? 4 : 1][4][64];
+    const SiteMu sm = site_mu_index(g);
+    const int a = sm.mu;
+    int y;
+    if (!stencil_site<(ST > 0)>(g, sm, y)) return;
+    int x[4];
+    decode(g, y, x);
+    const unsigned V = g.V;
+    const int lane = (int)(threadIdx.x & 63u);
+    const M3<T> dn = back_stencil_din(Din, V, a, y, rho, back_stencil_acc<T, ST>(g, U, Lc, y, x, a, lane, ul, hl));
     if constexpr (OUT < 0) {
         stm<T, GF_ST_NT>(Din, V, a, y, dn);
     } else {  // the force from the thin-link derivative
@@ -388,6 +405,92 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             if constexpr (OUT == FORCE_KICK_EXP) stu<T, GF_ST_NT>(Unew, V, a, y, exp_coeffs_times(p, eps_u, ua));
         }
     }
+}
+
+// K6a of level l-1 chained after the K6b of level l (stored C): the derivative with respect to
+// level l's input links, Dout_{l-1}(y), is formed in registers by the K6b part and is exactly what
+// the site-local step of level l-1 needs at the same (y, a), so it is neither stored nor read
+// back (1152 B per site fewer per level boundary).  Same arithmetic as k_stout_back_local<T,
+// true, false>: Din_{l-1} = Dout e^Q + (Lam C)^dag into Din (each thread reads then writes only its
+// own element), Lam_{l-1} into Lout (another buffer than Lc, whose entries at neighbour sites the
+// K6b part reads).  Dout and C are staged in the lane's LDS slice between their two uses, in LDS
+// that the K6b part's copies occupied before (one workgroup barrier in between).
+template <typename T>
+__device__ __forceinline__ void back_local_stored(const T* __restrict__ Uin, const T* __restrict__ Cst,
+                                                  const M3<T>& dout, T* __restrict__ Din, T* __restrict__ Lout,
+                                                  unsigned V, int mu, int s, int lane, MStage<T> dst, MStage<T> cst) {
+    M3<T> E, L;
+    {
+        const M3<T> u = full_of(ldr<T, GF_LD_NT>(Uin, V, mu, s));
+        const M3<T> Cq = stage_m3<T>(cst, mu, lane, ldm<T, GF_LD_NT>(Cst, V, mu, s));
+        exp_dexp(ta_mat(mul_nd(Cq, u)), mul(u, stage_m3<T>(dst, mu, lane, dout)), E, L);
+    }
+    const M3<T> Lam = ta_mat(L);
+    {
+        T c[8];
+        ta_coeffs(Lam, c);
+        stp<T, GF_ST_NT>(Lout, V, mu, s, c);
+    }
+    M3<T> D = mul(staged_m3<T>(dst, mu, lane), E);
+    acc_dag(D, mul(Lam, staged_m3<T>(cst, mu, lane)));
+    stm<T, GF_ST_NT>(Din, V, mu, s, D);
+}
+template <typename T>
+constexpr int kChainLds = (int)(2 * 4 * 9 * 64 * sizeof(T2<T>));  // dst + cst; >= ul + hl
+template <typename T, int ST>
+__global__ __launch_bounds__(kBlock) void k_stout_back_chain(Geo g, const T* __restrict__ U, const T* __restrict__ Lc,
+                                                             T* __restrict__ Din, double rho,
+                                                             const T* __restrict__ Uprev, const T* __restrict__ Cprev,
+                                                             T* __restrict__ Lout) {
+    __shared__ __align__(16) unsigned char smem[kChainLds<T>];
+    static_assert(4 * 10 * 64 * sizeof(T2<T>) <= kChainLds<T>, "ul + hl fit the staging space");
+    XShare<T> ul = reinterpret_cast<XShare<T>>(smem);
+    PairShare<T> hl = reinterpret_cast<PairShare<T>>(smem + 4 * 6 * 64 * sizeof(T2<T>));
+    MStage<T> dst = reinterpret_cast<MStage<T>>(smem);
+    MStage<T> cst = reinterpret_cast<MStage<T>>(smem + 4 * 9 * 64 * sizeof(T2<T>));
+    const SiteMu sm = site_mu_index(g);
+    const int a = sm.mu;
+    int y;
+    if (!stencil_site<(ST > 0)>(g, sm, y)) return;
+    int x[4];
+    decode(g, y, x);
+    const unsigned V = g.V;
+    const int lane = (int)(threadIdx.x & 63u);
+    const M3<T> dn = back_stencil_din(Din, V, a, y, rho, back_stencil_acc<T, ST>(g, U, Lc, y, x, a, lane, ul, hl));
+    if constexpr (ST > 0) __syncthreads();  // every wave is done with ul/hl before the staging reuses them
+    back_local_stored<T>(Uprev, Cprev, dn, Din, Lout, V, a, y, lane, dst, cst);
+}
+
+// K2 chained into the K6a of the top level (stored C): dS/dU of the top smeared level, coeff
+// Sigma_mu(y), is Dout of the top level's site-local step at the same (y, mu)
+template <typename T, bool SHARED>
+__global__ __launch_bounds__(kBlock) void k_dsdu_chain(Geo g, const T* __restrict__ Utop, double coeff,
+                                                       const T* __restrict__ Uin, const T* __restrict__ Cst,
+                                                       T* __restrict__ Din, T* __restrict__ Lout) {
+    __shared__ __align__(16) unsigned char smem[kChainLds<T>];
+    static_assert(6 * 6 * 64 * sizeof(T2<T>) <= kChainLds<T>, "xs fits the staging space");
+    XShare<T> xs = reinterpret_cast<XShare<T>>(smem);
+    MStage<T> dst = reinterpret_cast<MStage<T>>(smem);
+    MStage<T> cst = reinterpret_cast<MStage<T>>(smem + 4 * 9 * 64 * sizeof(T2<T>));
+    const SiteMu sm = site_mu_index(g);
+    int s;
+    if (!stencil_site<SHARED>(g, sm, s)) return;
+    int x[4];
+    decode(g, s, x);
+    M3<T> sig;
+    if constexpr (SHARED) {
+        R6<T> own;
+        sig = staple_sigma_shared(g, Utop, (unsigned)g.V, s, x, sm.mu, own, xs);
+    } else {
+        sig = staple_sigma(g, Utop, (unsigned)g.V, s, x, sm.mu);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        sig.m[k].r = scale1(coeff, sig.m[k].r);
+        sig.m[k].i = scale1(coeff, sig.m[k].i);
+    }
+    if constexpr (SHARED) __syncthreads();  // every wave is done with xs before the staging reuses it
+    back_local_stored<T>(Uin, Cst, sig, Din, Lout, (unsigned)g.V, sm.mu, s, (int)(threadIdx.x & 63u), dst, cst);
 }
 
 // ---------------------------------------------------------------------------
@@ -427,8 +530,31 @@ void launch_back_stencil(int form, int nb, hipStream_t st, const Geo& g, const T
         k_stout_back_stencil<T, ST_OWN, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
 }
 template <typename T>
-hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc, double rho,
-                                 hipStream_t st, int fmode, T* FP, double eps, T* Unew, double eps_u) {
+hipError_t launch_stout_back_chain(const Geo& g, const T* Uin, const T* Lc, T* Din, double rho, const T* Uprev,
+                                   const T* Cprev, T* Lout, hipStream_t st) {
+    const int nb = grid_sites_mu(g.V), form = stencil_form(g);
+    if (form == ST_XROW)
+        k_stout_back_chain<T, ST_XROW><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, Uprev, Cprev, Lout);
+    else if (form == ST_SHARED)
+        k_stout_back_chain<T, ST_SHARED><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, Uprev, Cprev, Lout);
+    else
+        k_stout_back_chain<T, ST_OWN><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, Uprev, Cprev, Lout);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_dsdu_chain(const Geo& g, const T* Utop, double coeff, const T* Uin, const T* Cst, T* Din, T* Lout,
+                             hipStream_t st) {
+    const int nb = grid_sites_mu(g.V);
+    if (g.V % 64 == 0)
+        k_dsdu_chain<T, true><<<nb, kBlock, 0, st>>>(g, Utop, coeff, Uin, Cst, Din, Lout);
+    else
+        k_dsdu_chain<T, false><<<nb, kBlock, 0, st>>>(g, Utop, coeff, Uin, Cst, Din, Lout);
+    return hipGetLastError();
+}
+// K6a alone (the top level of a backward pass that starts from a given dS/dU)
+template <typename T>
+hipError_t launch_stout_back_local(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc, double rho,
+                                   hipStream_t st) {
     const int nb = grid_sites_mu(g.V);
     if (Cst)
         k_stout_back_local<T, true, false><<<nb, kBlock, 0, st>>>(g, Uin, Cst, Dout, Din, Lc, rho);
@@ -436,9 +562,13 @@ hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const
         k_stout_back_local<T, false, true><<<nb, kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, Lc, rho);
     else
         k_stout_back_local<T, false, false><<<nb, kBlock, 0, st>>>(g, Uin, nullptr, Dout, Din, Lc, rho);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const int form = stencil_form(g);
+    return hipGetLastError();
+}
+// K6b alone (the thin-link level, or any level whose lower neighbour is not chained)
+template <typename T>
+hipError_t launch_stout_back_stencil(const Geo& g, const T* Uin, const T* Lc, T* Din, double rho, hipStream_t st,
+                                     int fmode, T* FP, double eps, T* Unew, double eps_u) {
+    const int nb = grid_sites_mu(g.V), form = stencil_form(g);
     if (fmode == FORCE_STORE)
         launch_back_stencil<T, FORCE_STORE>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
     else if (fmode == FORCE_KICK)
@@ -449,12 +579,26 @@ hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const
         launch_back_stencil<T, -1>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
     return hipGetLastError();
 }
+template <typename T>
+hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc, double rho,
+                                 hipStream_t st, int fmode, T* FP, double eps, T* Unew, double eps_u) {
+    hipError_t e = launch_stout_back_local<T>(g, Uin, Cst, Dout, Din, Lc, rho, st);
+    if (e != hipSuccess) return e;
+    return launch_stout_back_stencil<T>(g, Uin, Lc, Din, rho, st, fmode, FP, eps, Unew, eps_u);
+}
 
 #define GF_INST2(T)                                                                                       \
     template hipError_t launch_flow_stage<T>(const Geo&, const T*, T*, T*, double, double, hipStream_t);  \
     template hipError_t launch_stout_forward<T>(const Geo&, const T*, T*, T*, double, hipStream_t);       \
     template hipError_t launch_stout_backward<T>(const Geo&, const T*, const T*, const T*, T*, T*, double, \
-                                                 hipStream_t, int, T*, double, T*, double);
+                                                 hipStream_t, int, T*, double, T*, double);                \
+    template hipError_t launch_stout_back_local<T>(const Geo&, const T*, const T*, const T*, T*, T*, double, \
+                                                   hipStream_t);                                            \
+    template hipError_t launch_stout_back_stencil<T>(const Geo&, const T*, const T*, T*, double, hipStream_t, \
+                                                     int, T*, double, T*, double);                          \
+    template hipError_t launch_stout_back_chain<T>(const Geo&, const T*, const T*, T*, double, const T*,      \
+                                                   const T*, T*, hipStream_t);                              \
+    template hipError_t launch_dsdu_chain<T>(const Geo&, const T*, double, const T*, const T*, T*, T*, hipStream_t);
 
 GF_INST2(double)
 GF_INST2(float)

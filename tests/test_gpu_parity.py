@@ -182,7 +182,9 @@ def test_fused_md_update(Lat, oracle):
 
 def test_exp_large_and_tiny_momenta(Lat, oracle):
     L = Lat()
-    for scale in (3.0, 1e-9, 0.0):
+    # c1 ~ 2 scale^2 per link: 3.0 the closed form, 0.45 both sides of c1 = 3/4 within a wave,
+    # 0.2 and below the series
+    for scale in (3.0, 0.45, 0.2, 1e-9, 0.0):
         P = scale * gen.gaussian_momenta(D4, 17)
         L.set_cold()
         L.upload_momenta(P)
@@ -262,15 +264,18 @@ def test_stout_smear_and_force_parity(Lat, oracle, mode):
 
 # stencil forms of the backward stencil: x-rows from LDS (4^4), the site's own U and Lam from LDS
 # (12,4,4,4: NX does not divide 64), every load from memory (2,4,2,6: V not a multiple of 64)
+# rho 0.12 keeps every exponent's spectral radius below 1 (the series coefficients), rho 0.5 takes
+# part of the links past c1 = 3/4 (the closed form) within the same waves
+@pytest.mark.parametrize("rho", [0.12, 0.5])
 @pytest.mark.parametrize("dims", [D4, (12, 4, 4, 4), (2, 4, 2, 6)])
-def test_stout_backward_level_parity(Lat, oracle, dims):
+def test_stout_backward_level_parity(Lat, oracle, dims, rho):
     L = Lat(dims)
     U = gen.hot_start_links(dims, 42)
     D = gen.hot_start_links(dims, 43)  # any link-shaped derivative
     L.upload_links(U)
-    L.stout_config(1, 0.12, "store")
+    L.stout_config(1, rho, "store")
     L.stout_smear(download=False)
-    assert maxdiff(L.stout_backward(D), oracle.stout_backward(dims, U, 0.12, D)) < 1e-13
+    assert maxdiff(L.stout_backward(D), oracle.stout_backward(dims, U, rho, D)) < 1e-13
 
 
 # the fused 2MN sequence: the thin-link backward stencil forms the force and the link update, in
