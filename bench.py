@@ -607,7 +607,7 @@ def main():
         "site_updates_per_s": V / (ms_per_step * 1e-3),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": recorded_traffic("void gf::k_force<double, 2, 2>"),
+                     "traffic": recorded_traffic("gf::k_force<double, 2, 2>"),
                      "algorithmic_bytes_per_site": REF_BYTES["md_update_fused"],
                      "device_layout": {"bytes_per_site": B["md_update_fused"], "achieved": achieved_dev,
                                        "frac": achieved_dev / HBM_PEAK_GBS}},

@@ -41,7 +41,8 @@ import Gaugefields: gauge_configuration, measure_plaquette, gaussian_momenta!, g
     md_action_workspace, md_potential, md_force!, md_hamiltonian, update_gaugefields!, update_momenta!,
     md_trajectory!, md_step!, md_step_size, gradient_flow
 import Gaugefields.AbstractGaugefields_module: Gaugefields_4D, TA_Gaugefields_4D, calculate_Plaquette,
-    substitute_U!, set_wing_U!, clear_U!, initialize_TA_Gaugefields, gauss_distribution!, topological_charge
+    substitute_U!, set_wing_U!, clear_U!, initialize_TA_Gaugefields, gauss_distribution!, topological_charge,
+    topological_charge_density
 import Gaugefields.GaugeAction_module: evaluate_GaugeAction, calc_dSdUμ!
 import Gaugefields.Gradientflow_module: flow!
 import Gaugefields.Abstractsmearing_module: calc_smearedU, back_prop, forward!, layer_pullback!
@@ -50,7 +51,8 @@ import Gaugefields.ILDG_format: load_gaugefield!, save_binarydata
 import Wilsonloop: make_loops_fromname
 
 export HIPBackend, Gaugefields_4D_HIP, TA_Gaugefields_4D_HIP, OmelyanPQP, StoutGaugeAction,
-    stout_smear, stout_smear!, stout_pullback, force_field, backup
+    stout_smear, stout_smear!, stout_pullback, force_field, backup, calculate_energy_density, energy_density,
+    flow_measure!
 
 const libgfhip = joinpath(@__DIR__, "..", "libgfhip.so")
 
@@ -304,6 +306,48 @@ function topological_charge(U::HIPLinks; method=:plaquette)
     _check(c, ccall((:gf_topological_charge, libgfhip), Cint, (Ptr{Cvoid}, Cint, Ref{Float64}),
                     _ctx(c), GF_TOPO_METHODS[method], Q))
     return Q[]
+end
+
+"""topological_charge_density(U; method) (AbstractGaugefields.jl:1430-1446): q(x) as an
+Array{Float64,4}(NX, NY, NZ, NT)."""
+function topological_charge_density(U::HIPLinks; method=:plaquette)
+    haskey(GF_TOPO_METHODS, method) ||
+        throw(ArgumentError("supported topological_charge_density methods are :plaquette, :clover, and :improved"))
+    c = _context(U)
+    q = Vector{Float64}(undef, prod(c.dims))
+    _check(c, ccall((:gf_topological_charge_density, libgfhip), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}),
+                    _ctx(c), GF_TOPO_METHODS[method], q))
+    return reshape(q, c.dims...)
+end
+
+"""The clover energy density of samples/measurements/energydensity.jl:32-76 (calculate_energy_density)."""
+function calculate_energy_density(U::HIPLinks)
+    c = _context(U)
+    E = Ref{Float64}(0)
+    _check(c, ccall((:gf_energy_density_clover, libgfhip), Cint, (Ptr{Cvoid}, Ref{Float64}), _ctx(c), E))
+    return E[]
+end
+
+"""The plaquette energy density E = 36 (1 - plaquette) (Lüscher, JHEP 08 (2010) 071)."""
+function energy_density(U::HIPLinks)
+    c = _context(U)
+    E = Ref{Float64}(0)
+    _check(c, ccall((:gf_energy_density, libgfhip), Cint, (Ptr{Cvoid}, Ref{Float64}), _ctx(c), E))
+    return E[]
+end
+
+"""flow_measure!(U, eps, nsteps, every): nsteps RK3 flow steps with, every `every` steps, the
+measurements of samples/measurements/topologicalcharge.jl:236-245 and energydensity.jl:100-118:
+a (nsteps ÷ every) × 7 matrix of rows (t, plaquette/(18V), E plaquette, E clover, Q plaquette,
+Q clover, Q improved)."""
+function flow_measure!(U::HIPLinks, eps::Real, nsteps::Integer, every::Integer)
+    every > 0 || throw(ArgumentError("every must be positive; got $every"))
+    c = _context(U)
+    nrow = nsteps ÷ every
+    out = Vector{Float64}(undef, 7 * nrow)
+    _check(c, ccall((:gf_flow_measure, libgfhip), Cint, (Ptr{Cvoid}, Cdouble, Cint, Cint, Ptr{Float64}),
+                    _ctx(c), Float64(eps), Cint(nsteps), Cint(every), out))
+    return permutedims(reshape(out, 7, nrow))
 end
 
 # ----------------------------------------------------------------------------- Wilson action

@@ -259,6 +259,30 @@ def jl_topological_charge(R, c, method):
     return Q.value
 
 
+def jl_topological_charge_density(R, c, method):
+    q = np.empty(c.V)
+    _ok(R.gf_topological_charge_density(c.ptr, TOPO[method], _d(q)))
+    return q
+
+
+def jl_calculate_energy_density(R, c):  # clover
+    E = ctypes.c_double(0)
+    _ok(R.gf_energy_density_clover(c.ptr, ctypes.byref(E)))
+    return E.value
+
+
+def jl_energy_density(R, c):  # plaquette
+    E = ctypes.c_double(0)
+    _ok(R.gf_energy_density(c.ptr, ctypes.byref(E)))
+    return E.value
+
+
+def jl_flow_measure(R, c, eps, nsteps, every):
+    out = np.empty(7 * (nsteps // every))
+    _ok(R.gf_flow_measure(c.ptr, eps, nsteps, every, _d(out)))
+    return out.reshape(nsteps // every, 7)
+
+
 def jl_backup(R, c):  # substitute_U!(b::HIPBackup, U)
     _ok(R.gf_save_links(c.ptr))
 
@@ -327,6 +351,10 @@ REPLAYS = [
     ("function stout_smear(", jl_stout_smear, (2, 0.1, AUTO, ZL), True),
     ("function stout_pullback(", jl_stout_pullback, (2, 0.1, AUTO, ZL, ZL), True),
     ("function topological_charge(", jl_topological_charge, ("clover",), True),
+    ("function topological_charge_density(", jl_topological_charge_density, ("improved",), True),
+    ("function calculate_energy_density(", jl_calculate_energy_density, (), True),
+    ("function energy_density(", jl_energy_density, (), True),
+    ("function flow_measure!(", jl_flow_measure, (0.01, 4, 2), True),
     ("function substitute_U!(b::HIPBackup", jl_backup, (), True),
     ("function substitute_U!(U::HIPLinks, b::HIPBackup)", jl_restore, (), True),
     ("function heatbath!(", jl_heatbath, (6.0, 3, 0, 100000), True),
@@ -475,6 +503,18 @@ def test_replays_against_the_oracle(gf, oracle, tmp_path):
         jl_substitute_links(R, c, U)
         qref = oracle.topological_charge_density(dims, U, "clover")[0]
         assert abs(jl_topological_charge(R, c, "clover") - qref) < 1e-12
+        Qi, qi = oracle.topological_charge_density(dims, U, "improved")
+        assert np.abs(jl_topological_charge_density(R, c, "improved") - qi).max() < 1e-12
+        Ec = oracle.energy_density_clover(dims, U)
+        assert abs(jl_calculate_energy_density(R, c) - Ec) < 1e-12 * abs(Ec)
+        Ep = oracle.energy_density(dims, U)
+        assert abs(jl_energy_density(R, c) - Ep) < 1e-11
+        rows = jl_flow_measure(R, c, 0.01, 4, 2)
+        Uf2 = U.copy()
+        oracle.flow(dims, Uf2, 0.01, 2)
+        assert rows.shape == (2, 7) and abs(rows[0, 0] - 0.02) < 1e-15
+        assert abs(rows[0, 2] - oracle.energy_density(dims, Uf2)) < 1e-11
+        jl_substitute_links(R, c, U)
         Uh = U.copy()
         fails = jl_heatbath(R, c, 6.0, 3, 0, 100000)
         oracle.heatbath_sweep(dims, Uh, 6.0, 3, 0, 65534)
