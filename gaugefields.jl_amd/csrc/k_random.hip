@@ -253,6 +253,22 @@ __device__ __forceinline__ bool hb_subgroup(M3<double>& u, const M3<double>& St,
     return true;
 }
 
+// Half-volume index of a heatbath thread.  Half-site h is the pair of sites (2h, 2h+1) (one of
+// each colour, NX even), so the 64-site group k holds half-sites 32k .. 32k+31.  With the
+// t-sweep tiles of the stencil kernels (V a multiple of 4096: 8 groups per workgroup, a grid
+// that is a multiple of 8), lanes 0-31 and 32-63 of wave w of block b take the groups the
+// stencil kernels give to blocks 8b + 2w and 8b + 2w + 1: the sites in flight on an XCD then
+// sweep its tile slice by slice in t, and the neighbour links a staple re-reads are still in
+// that XCD's L2.  Otherwise linear order.
+__device__ __forceinline__ int heatbath_half_site(const Geo& g) {
+    if (g.tile[0] > 0 && (unsigned)g.V == gridDim.x * 512u && (gridDim.x & 7u) == 0u) {
+        const unsigned lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+        const int grp = tsweep_group(g, xcd_block() * 8u + 2u * w + (lane >> 5), gridDim.x * 8u);
+        return grp * 32 + (int)(lane & 31u);
+    }
+    return (int)(xcd_block() * blockDim.x + threadIdx.x);
+}
+
 // site of colour `par` with half-volume index h (NX even): x = 2 (h mod NX/2) + ((y+z+t+par) & 1)
 __device__ __forceinline__ int colour_site(const Geo& g, int h, int par, int x[4]) {
     const int nxh = g.n[0] >> 1;
@@ -272,17 +288,23 @@ __device__ __forceinline__ int colour_site(const Geo& g, int h, int par, int x[4
 // (SUN_overrelaxation_rng!, heatbathmodule.jl:1295-1321, subgroup i drawn from block i of tag 6:
 // n = floor(2 w0 / 2^32), k = n + 1 + floor((2 - n) w1 / 2^32)).
 template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_heatbath(Geo g, T* __restrict__ U, int mu, int par, double beta,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_heatbath(Geo g, T* __restrict__ U, int mu, int par, double beta,
                                                      unsigned long long seed, unsigned long long sweep, int itmax,
                                                      double* __restrict__ partial) {
-    const int h = (int)(xcd_block() * blockDim.x + threadIdx.x);
+    const int h = heatbath_half_site(g);
     double fail = 0.0;
     if (h < (g.V >> 1)) {
         int x[4];
         const int s = colour_site(g, h, par, x);
         const unsigned V = g.V;
-        const M3<double> St = to_f64(staple_sigma(g, (const T*)U, V, s, x, mu));
-        M3<double> u = to_f64(ldu((const T*)U, V, mu, (unsigned)s));
+        R6<T> own;
+        M3<double> St = to_f64(staple_sigma_own(g, (const T*)U, V, s, x, mu, own));
+        // pin the staple sum: otherwise the compiler interleaves the staple loads with the
+        // sampler and needs 256 VGPRs
+#pragma unroll
+        for (int e = 0; e < 9; e++) asm volatile("" : "+v"(St.m[e].r), "+v"(St.m[e].i));
+        __builtin_amdgcn_sched_barrier(0);
+        M3<double> u = to_f64(full_of(own));
         bool ok = true;
         if (MODE == 0) {
             ok = hb_subgroup<0, 1>(u, St, seed, sweep, (unsigned)s, mu, 3, beta, itmax) &&

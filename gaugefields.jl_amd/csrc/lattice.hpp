@@ -770,8 +770,8 @@ __device__ __forceinline__ bool stencil_site(const Geo& g, const SiteMu& sm, int
     }
 }
 
-// Unpipelined form for kernels with large register state of their own (heatbath, the stout
-// backward's recomputation, dS/dU), where the prefetch buffer would cost occupancy: links
+// Unpipelined form for kernels with large register state of their own (the stout backward's
+// recomputation, dS/dU off the 64-site grid), where the prefetch buffer would cost occupancy: links
 // are loaded at their use, through opaque copies of the site indices so that the compiler
 // neither merges them with the loads of other (mu, nu) iterations nor hoists them all.
 template <typename T>

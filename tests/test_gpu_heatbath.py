@@ -46,8 +46,10 @@ def test_heatbath_matches_oracle(Lat, oracle, dims, beta):
     assert L.plaquette_sum() == pytest.approx(oracle.plaquette_sum(dims, W), rel=1e-11)
 
 
-def test_overrelaxation_matches_oracle_and_keeps_action(Lat, oracle):
-    dims = (4, 4, 4, 4)
+# 8^4: the tiled thread order (V a multiple of 4096); 4^4: linear order
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (8, 8, 8, 8)])
+def test_overrelaxation_matches_oracle_and_keeps_action(Lat, oracle, dims):
+    print(f"[overrelaxation] {dims}", flush=True)
     U = oracle.hot_links(dims, 18)
     W = U.copy()
     for sw in range(3):
