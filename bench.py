@@ -79,8 +79,11 @@ def make_lattice(gf, dims, precision, seed_u=20260, seed_p=20261, sigma=0.3):
     return L
 
 
-def time_kernel(L, fn, reps):
-    """average duration (ms) of fn() over reps launches, HIP events on the library stream"""
+def time_kernel(L, fn, reps, warmup=1):
+    """average duration (ms) of fn() over reps launches after `warmup` untimed ones, HIP events
+    on the library stream"""
+    for _ in range(warmup):
+        fn()
     L.sync()
     L.event_record(2)
     for _ in range(reps):

@@ -341,17 +341,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     block_sum_store<kBlock>(fail, partial);
 }
 
+// Dynamic LDS (unused) of an FP64 heatbath launch: 56 KB per workgroup leaves room for two
+// workgroups per CU instead of the three the 168 VGPRs allow.  A heatbath thread owns one site of
+// one colour, so the sites in flight per XCD are eight times the stencil kernels'; with a third
+// fewer of them the links their staples share stay in L2 between uses (32^4: sweep 1.81 -> 1.52 ms,
+// overrelaxation 1.61 -> 1.42 ms).  FP32 links are half the bytes and keep three workgroups.
+constexpr size_t kHeatbathOccLds = 56 * 1024;
+
 template <typename T>
 hipError_t launch_heatbath(const Geo& g, T* U, int mode, double beta, unsigned long long seed,
                            unsigned long long sweep, int itmax, double* partial, double* failures, hipStream_t st) {
     const int nb = grid_for(g.V / 2);
+    const size_t lds = sizeof(T) == 8 ? kHeatbathOccLds : 0;
     for (int mu = 0; mu < 4; mu++)
         for (int par = 0; par < 2; par++) {
             double* part = partial + (size_t)(2 * mu + par) * nb;
             if (mode == 0)
-                k_heatbath<T, 0><<<nb, kBlock, 0, st>>>(g, U, mu, par, beta, seed, sweep, itmax, part);
+                k_heatbath<T, 0><<<nb, kBlock, lds, st>>>(g, U, mu, par, beta, seed, sweep, itmax, part);
             else
-                k_heatbath<T, 1><<<nb, kBlock, 0, st>>>(g, U, mu, par, beta, seed, sweep, itmax, part);
+                k_heatbath<T, 1><<<nb, kBlock, lds, st>>>(g, U, mu, par, beta, seed, sweep, itmax, part);
         }
     hipError_t err = hipGetLastError();
     if (err == hipSuccess) err = launch_sum_partials(partial, 8 * nb, failures, st);
