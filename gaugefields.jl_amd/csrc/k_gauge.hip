@@ -29,11 +29,62 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(Geo g, const T* __restrict
             const int mu = pl < 3 ? 0 : (pl < 5 ? 1 : 2);
             const int nu = pl < 3 ? pl + 1 : (pl < 5 ? pl - 1 : 3);
             const int spm = s + dlt(g, x, mu, 1), spn = s + dlt(g, x, nu, 1);
-            // products in FP64 also for FP32 fields: the action/Hamiltonian sums carry no float rounding
-            const M3<double> a = mul(to_f64(ldu(U, V, mu, s)), to_f64(ldu(U, V, nu, spm)));  // U_mu(x) U_nu(x+mu)
-            const M3<double> b = mul(to_f64(ldu(U, V, nu, s)), to_f64(ldu(U, V, mu, spn)));  // U_nu(x) U_mu(x+nu)
+            // products in FP64 also for FP32 fields (the stored rows widened, row 2 rebuilt in double):
+            // the action/Hamiltonian sums carry no float rounding
+            auto ld = [&](int d, int t) { return full_of(r6_f64(ldr(U, V, d, (unsigned)t))); };
+            const M3<double> a = mul(ld(mu, s), ld(nu, spm));  // U_mu(x) U_nu(x+mu)
+            const M3<double> b = mul(ld(nu, s), ld(mu, spn));  // U_nu(x) U_mu(x+nu)
             v += retr_nd(a, b);
         }
+    }
+    block_sum_store<kBlock>(v, partial);
+}
+
+// K1 for workgroups of whole x-rows (NX divides 64, V a multiple of 64): each wave stores its
+// U_w(x) in LDS, so the 13 distinct links the six planes of a site need are loaded once per
+// workgroup (4 own links, U_0(x+nu) of the planes (0, nu); the links at x+0 from the x-row)
+// instead of 4 per plane (24).  Waves 0..2 take the planes (0, w+1) and (1,2) / (1,3) / (2,3):
+// three loaded links each; wave 3 loads only its own link.  Same products in FP64 as k_plaquette.
+// 32^4 FP64: 0.190 -> 0.177 ms per plaquette with the host readback.
+template <typename T>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_plaquette_xrow(
+    Geo g, const T* __restrict__ U, double* __restrict__ partial) {
+    __shared__ T2<T> ol[4][6][64];
+    const SiteMu sm = site_mu_index(g);
+    const int s = sm.s, w = sm.mu;  // V % 64 == 0: every lane is a site
+    const int lane = (int)(threadIdx.x & 63u);
+    int x[4];
+    decode(g, s, x);
+    const unsigned V = g.V;
+    auto lds_link = [&](int d, int l) {
+        R6<T> r;
+#pragma unroll
+        for (int e = 0; e < 6; e++) r.v[e] = ol[d][e][l];
+        return full_of(r6_f64(r));
+    };
+    {
+        const R6<T> o = ldr(U, V, w, (unsigned)s);
+#pragma unroll
+        for (int e = 0; e < 6; e++) ol[w][e][lane] = o.v[e];
+    }
+    // the wave's memory links, issued before the barrier: plane (0, n1) needs U_0(x+n1); plane
+    // (m2, n2) needs U_n2(x+m2) and U_m2(x+n2)
+    const int n1 = w + 1;
+    const int m2 = w == 2 ? 2 : 1, n2 = w == 0 ? 2 : 3;
+    R6<T> l1, l2, l3;
+    if (w < 3) {
+        l1 = ldr(U, V, 0, (unsigned)(s + dlt(g, x, n1, 1)));
+        l2 = ldr(U, V, n2, (unsigned)(s + dlt(g, x, m2, 1)));
+        l3 = ldr(U, V, m2, (unsigned)(s + dlt(g, x, n2, 1)));
+    }
+    __syncthreads();  // every wave's U(x) is in LDS
+    double v = 0.0;
+    if (w < 3) {
+        const int lxp = xlane(lane, g.n[0] - 1, 1);
+        // Re tr U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag = Re tr (a b^dag)
+        v += retr_nd(mul(lds_link(0, lane), lds_link(n1, lxp)), mul(lds_link(n1, lane), full_of(r6_f64(l1))));
+        asm volatile("" : "+v"(v));  // the first plane before the second one's LDS reads
+        v += retr_nd(mul(lds_link(m2, lane), full_of(r6_f64(l2))), mul(lds_link(n2, lane), full_of(r6_f64(l3))));
     }
     block_sum_store<kBlock>(v, partial);
 }
@@ -208,7 +259,9 @@ __global__ __launch_bounds__(kBlock) void k_links_to_ref(const T* __restrict__ s
     if (s >= V) return;
     for (int mu = 0; mu < 4; mu++) {
         double* r = ref + ((size_t)mu * V + s) * 18;
-        const M3<T> m = GEN ? ldm(soa, (unsigned)V, mu, (unsigned)s) : ldu(soa, (unsigned)V, mu, (unsigned)s);
+        // SU(3) links: the stored rows widened, row 2 rebuilt in double (as the FP64 sums use them)
+        const M3<double> m = GEN ? to_f64(ldm(soa, (unsigned)V, mu, (unsigned)s))
+                                 : full_of(r6_f64(ldr(soa, (unsigned)V, mu, (unsigned)s)));
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -331,7 +384,10 @@ hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStr
 template <typename T>
 hipError_t launch_plaquette(const Geo& g, const T* U, double* partial, double* out, hipStream_t st) {
     const int nb = grid_sites_mu(g.V);
-    k_plaquette<T><<<nb, kBlock, 0, st>>>(g, U, partial);
+    if (sizeof(T) == 8 && stencil_form(g) == ST_XROW)  // FP32: 16 B spilled at 168 VGPRs, slower
+        k_plaquette_xrow<T><<<nb, kBlock, 0, st>>>(g, U, partial);
+    else
+        k_plaquette<T><<<nb, kBlock, 0, st>>>(g, U, partial);
     k_sum_partials<<<1, 1024, 0, st>>>(partial, nb, out);
     return hipGetLastError();
 }

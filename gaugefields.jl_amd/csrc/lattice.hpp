@@ -262,6 +262,18 @@ __device__ __forceinline__ M3<T> full_of(const R6<T>& r) {
     complete_row2(m);
     return m;
 }
+// the two stored rows in double (exact); full_of() of it rebuilds row 2 in double, the same for
+// every kernel that sums in FP64 over FP32-stored links
+template <typename T>
+__device__ __forceinline__ R6<double> r6_f64(const R6<T>& r) {
+    R6<double> o;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        o.v[k].x = (double)r.v[k].x;
+        o.v[k].y = (double)r.v[k].y;
+    }
+    return o;
+}
 template <typename T, int AUX = GF_LD_DEFAULT>
 __device__ __forceinline__ M3<T> ldu(const T* __restrict__ p, unsigned V, int mu, unsigned s) {
     return full_of(ldr<T, AUX>(p, V, mu, s));
