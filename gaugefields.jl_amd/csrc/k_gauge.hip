@@ -71,16 +71,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     // (m2, n2) needs U_n2(x+m2) and U_m2(x+n2)
     const int n1 = w + 1;
     const int m2 = w == 2 ? 2 : 1, n2 = w == 0 ? 2 : 3;
-    R6<T> l1, l2, l3;
-    if (w < 3) {
-        l1 = ldr(U, V, 0, (unsigned)(s + dlt(g, x, n1, 1)));
-        l2 = ldr(U, V, n2, (unsigned)(s + dlt(g, x, m2, 1)));
-        l3 = ldr(U, V, m2, (unsigned)(s + dlt(g, x, n2, 1)));
-    }
+    R6<T> l1;
+    if (w < 3) l1 = ldr(U, V, 0, (unsigned)(s + dlt(g, x, n1, 1)));
     __syncthreads();  // every wave's U(x) is in LDS
     double v = 0.0;
     if (w < 3) {
         const int lxp = xlane(lane, g.n[0] - 1, 1);
+        // the second plane's links, in flight during the first plane's products
+        const R6<T> l2 = ldr(U, V, n2, (unsigned)(s + dlt(g, x, m2, 1)));
+        const R6<T> l3 = ldr(U, V, m2, (unsigned)(s + dlt(g, x, n2, 1)));
         // Re tr U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag = Re tr (a b^dag)
         v += retr_nd(mul(lds_link(0, lane), lds_link(n1, lxp)), mul(lds_link(n1, lane), full_of(r6_f64(l1))));
         asm volatile("" : "+v"(v));  // the first plane before the second one's LDS reads
