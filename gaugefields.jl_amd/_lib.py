@@ -68,6 +68,7 @@ SIGNATURES = {
     "gf_stout_backward": (ctypes.c_int, [_CTX, _D, _D]),
     "gf_stout_force": (ctypes.c_int, [_CTX, ctypes.c_double]),
     "gf_stout_force_kick": (ctypes.c_int, [_CTX, ctypes.c_double, ctypes.c_double]),
+    "gf_stout_force_add": (ctypes.c_int, [_CTX, ctypes.c_double]),
     "gf_ildg_info": (ctypes.c_int, [ctypes.c_char_p, _I, _I, ctypes.POINTER(ctypes.c_longlong)]),
     "gf_ildg_read": (ctypes.c_int, [ctypes.c_char_p, _I, _D, _I]),
     "gf_ildg_write": (ctypes.c_int, [ctypes.c_char_p, _I, _D, ctypes.c_int]),

@@ -293,29 +293,98 @@ class _Workspace:
     pass
 
 
+class MDActionSet:
+    """Independently scheduled MD action providers by name (MDActionSet, molecular_dynamics.jl:45-68):
+    the potential and force are the sums of the members'.  Members are GaugeAction or
+    SmearedGaugeAction providers; the names select them in MDForceGroup / SextonWeingarten."""
+
+    def __init__(self, terms=None, **kw):
+        terms = dict(terms or {}, **kw)
+        if not terms:
+            raise ArgumentError("an MDActionSet must contain at least one action provider")
+        self.terms = terms
+
+    def keys(self):
+        return tuple(self.terms)
+
+
+class MDForceGroup:
+    """Named members of an MDActionSet for a momentum update (MDForceGroup, molecular_dynamics.jl:70-93)."""
+
+    def __init__(self, *names):
+        if len(names) == 1 and isinstance(names[0], (tuple, list)):
+            names = tuple(names[0])
+        if not names:
+            raise ArgumentError("an MDForceGroup must contain at least one action name")
+        if len(set(names)) != len(names):
+            raise ArgumentError(f"an MDForceGroup must not contain duplicate action names: {tuple(names)}")
+        self.names = tuple(names)
+
+
+def _md_force_group(x):
+    if isinstance(x, MDForceGroup):
+        return x
+    if isinstance(x, str):
+        return MDForceGroup(x)
+    if isinstance(x, (tuple, list)):
+        return MDForceGroup(*x)
+    raise ArgumentError(f"a force group is an action name, a tuple of names or an MDForceGroup; got {x!r}")
+
+
+def _providers(action, group=None):
+    """the (provider) members a force evaluation sums: the action itself, or the set's members
+    (all, or those named by group, in the group's order)"""
+    if isinstance(action, MDActionSet):
+        names = group.names if group is not None else action.keys()
+        return [action.terms[n] for n in names]
+    if group is not None:
+        raise ArgumentError(f"a force group selects members of an MDActionSet; got {type(action).__name__}")
+    return [action]
+
+
+def _configure_member(member, U):
+    """the device's stout setting for one provider; returns its plaquette coefficient"""
+    if isinstance(member, SmearedGaugeAction):
+        member.configure(U)
+    elif isinstance(member, GaugeAction):
+        U.lattice.stout_config(0, 0.0)
+    else:
+        raise ArgumentError(f"md_force! is not implemented for {type(member).__name__}")
+    return member.coefficient
+
+
 def md_action_workspace(action, U):
-    if not isinstance(action, (GaugeAction, SmearedGaugeAction)):
-        raise ArgumentError(f"md_action_workspace is not implemented for {type(action).__name__}")
+    for m in _providers(action):
+        if not isinstance(m, (GaugeAction, SmearedGaugeAction)):
+            raise ArgumentError(f"md_action_workspace is not implemented for {type(m).__name__}")
     return _Workspace()
 
 
+def _member_potential(member, U):
+    if isinstance(member, SmearedGaugeAction):
+        return member.potential(U)
+    if not isinstance(member, GaugeAction):
+        raise ArgumentError(f"md_potential is not implemented for {type(member).__name__}")
+    return -evaluate_GaugeAction(member, U) / U.NC
+
+
 def md_potential(action, U, workspace=None):
-    if isinstance(action, SmearedGaugeAction):
-        return action.potential(U)
-    if not isinstance(action, GaugeAction):
-        raise ArgumentError(f"md_potential is not implemented for {type(action).__name__}")
-    return -evaluate_GaugeAction(action, U) / U.NC
+    """-Re S / NC of the action (molecular_dynamics.jl:247-249); the sum over the members of an
+    MDActionSet (:111-126)"""
+    return sum(_member_potential(m, U) for m in _providers(action))
 
 
-def md_force(force, action, U, workspace=None):
-    """Write dp/dtau into the configuration's force field; returns it as (4, V, 8)."""
-    if isinstance(action, SmearedGaugeAction):
-        action.configure(U)
-    elif isinstance(action, GaugeAction):
-        U.lattice.stout_config(0, 0.0)
-    else:
-        raise ArgumentError(f"md_force! is not implemented for {type(action).__name__}")
-    U.lattice.force(action.coefficient)
+def md_force(force, action, U, workspace=None, group=None):
+    """Write dp/dtau into the configuration's force field; returns it as (4, V, 8).  For an
+    MDActionSet the sum of the members' forces, or of those the MDForceGroup `group` names
+    (molecular_dynamics.jl:201-236): the first member's force written, every further one added."""
+    members = _providers(action, group)
+    for k, m in enumerate(members):
+        coeff = _configure_member(m, U)
+        if k == 0:
+            U.lattice.force(coeff)
+        else:
+            U.lattice.force_add(coeff)
     F = U.lattice.download_force()
     if force is not None:
         force[...] = F
@@ -346,6 +415,43 @@ class OmelyanPQP:
         self.lam = float(lam)
 
 
+class SextonWeingarten:
+    """Second-order Sexton-Weingarten integrator for two force time scales (molecular_dynamics.jl:
+    298-360): slow and fast are action names, tuples of names or MDForceGroups of an MDActionSet.
+    QPQ ordering: a fast evolution of half the step (n_fast QPQ substeps) on each side of one full
+    slow kick; PQP ordering: half slow kicks around one full fast evolution."""
+
+    def __init__(self, *, slow, fast, n_fast, ordering=None):
+        ordering = QPQ() if ordering is None else ordering
+        if int(n_fast) <= 0:
+            raise ArgumentError(f"n_fast must be positive; got {n_fast}")
+        if not isinstance(ordering, (PQP, QPQ)):
+            raise ArgumentError(f"SextonWeingarten ordering must be PQP() or QPQ(); got {type(ordering).__name__}")
+        self.slow = _md_force_group(slow)
+        self.fast = _md_force_group(fast)
+        self.n_fast = int(n_fast)
+        self.ordering = ordering
+
+
+def _validate_md_integrator(integrator, action):
+    """molecular_dynamics.jl:362-407"""
+    if not isinstance(integrator, SextonWeingarten):
+        return
+    if not isinstance(action, MDActionSet):
+        raise ArgumentError(f"SextonWeingarten requires an MDActionSet; got {type(action).__name__}")
+    available = action.keys()
+    slow, fast = integrator.slow.names, integrator.fast.names
+    overlap = tuple(n for n in slow if n in fast)
+    if overlap:
+        raise ArgumentError(f"slow and fast force groups overlap: {overlap}")
+    unknown = tuple(n for n in slow + fast if n not in available)
+    if unknown:
+        raise ArgumentError(f"SextonWeingarten selects unknown actions {unknown}; available names are {available}")
+    omitted = tuple(n for n in available if n not in slow + fast)
+    if omitted:
+        raise ArgumentError(f"SextonWeingarten does not schedule actions {omitted}")
+
+
 class MDDriver:
     def __init__(self, action, integrator, trajectory_length, steps):
         self.action = action
@@ -369,8 +475,10 @@ def md_driver(U, action, *, steps, trajectory_length=1.0, integrator=None):
         if tl32 == 0:
             raise ArgumentError("trajectory_length becomes zero after conversion to Float32")
         trajectory_length = tl32
+    integrator = integrator if integrator is not None else QPQ()
+    _validate_md_integrator(integrator, action)
     md_action_workspace(action, U)
-    return MDDriver(action, integrator if integrator is not None else QPQ(), trajectory_length, int(steps))
+    return MDDriver(action, integrator, trajectory_length, int(steps))
 
 
 def md_step_size(driver):
@@ -378,16 +486,14 @@ def md_step_size(driver):
 
 
 def _configure(driver, U):
-    a = driver.action
-    if isinstance(a, SmearedGaugeAction):
-        a.configure(U)
-    else:
-        U.lattice.stout_config(0, 0.0)
-    return a.coefficient
+    return _configure_member(driver.action, U)
 
 
 def md_hamiltonian(U, P, driver):
+    """md_potential + p*p/2 (molecular_dynamics.jl:494-505)"""
     _check_same(U, P)
+    if isinstance(driver.action, MDActionSet):
+        return md_potential(driver.action, U) + 0.5 * U.lattice.momenta_dot()
     return U.lattice.hamiltonian(_configure(driver, U))
 
 
@@ -400,19 +506,41 @@ def update_gaugefields(U, P, step_size, driver):
     return U
 
 
-def update_momenta(P, U, step_size, driver):
-    """P <- P + step_size * force(U)"""
+def update_momenta(P, U, step_size, driver, group=None):
+    """P <- P + step_size * force(U) (molecular_dynamics.jl:533-551); with an MDForceGroup, only
+    the named members of the driver's MDActionSet (:553-583).  Each member's force and kick are
+    one library call."""
     _check_same(U, P)
     if not math.isfinite(step_size):
         raise ArgumentError(f"the momentum step size must be finite; got {step_size}")
-    U.lattice.force_kick(_configure(driver, U), step_size)
+    for m in _providers(driver.action, group):
+        U.lattice.force_kick(_configure_member(m, U), step_size)
     return P
 
 
+def _md_fast_qpq(U, P, duration, nsteps, driver, fast):
+    """n QPQ substeps of the fast group over `duration`, adjacent half link updates merged
+    (_md_fast_qpq!, molecular_dynamics.jl:618-634)"""
+    step_size = duration / nsteps
+    update_gaugefields(U, P, step_size / 2, driver)
+    for step in range(1, nsteps + 1):
+        update_momenta(P, U, step_size, driver, fast)
+        update_gaugefields(U, P, step_size / 2 if step == nsteps else step_size, driver)
+
+
 def md_step(integrator, U, P, step_size, driver):
-    if callable(integrator) and not isinstance(integrator, (QPQ, PQP, OmelyanPQP)):
+    if callable(integrator) and not isinstance(integrator, (QPQ, PQP, OmelyanPQP, SextonWeingarten)):
         return integrator(U, P, step_size, driver)
-    if isinstance(integrator, PQP):
+    if isinstance(integrator, SextonWeingarten):  # molecular_dynamics.jl:636-700
+        if isinstance(integrator.ordering, QPQ):
+            _md_fast_qpq(U, P, step_size / 2, integrator.n_fast, driver, integrator.fast)
+            update_momenta(P, U, step_size, driver, integrator.slow)
+            _md_fast_qpq(U, P, step_size / 2, integrator.n_fast, driver, integrator.fast)
+        else:
+            update_momenta(P, U, step_size / 2, driver, integrator.slow)
+            _md_fast_qpq(U, P, step_size, integrator.n_fast, driver, integrator.fast)
+            update_momenta(P, U, step_size / 2, driver, integrator.slow)
+    elif isinstance(integrator, PQP):
         update_momenta(P, U, step_size / 2, driver)
         update_gaugefields(U, P, step_size, driver)
         update_momenta(P, U, step_size / 2, driver)
@@ -439,7 +567,7 @@ def md_trajectory(U, P, driver, *, diagnostics=True, fused=False):
     Custom integrators (callables) are stepped from here."""
     _check_same(U, P)
     integ = driver.integrator
-    if isinstance(integ, (QPQ, PQP, OmelyanPQP)):
+    if isinstance(integ, (QPQ, PQP, OmelyanPQP)) and not isinstance(driver.action, MDActionSet):
         coeff = _configure(driver, U)
         lam = integ.lam if isinstance(integ, OmelyanPQP) else OMELYAN_LAMBDA
         H = U.lattice.md_trajectory(coeff, driver.steps, driver.trajectory_length, integ.code, lam=lam,

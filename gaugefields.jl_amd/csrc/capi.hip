@@ -331,10 +331,12 @@ int stout_backward_all(gf_ctx* c, int fmode = -1, void* FP = nullptr, double eps
 
 // mode FORCE_STORE -> F, FORCE_KICK -> P += eps F, FORCE_KICK_EXP -> that and Ualt = exp(eps_u P) U
 // (the caller swaps the link buffers)
-int stout_force_impl(gf_ctx* c, double coeff, int mode, double eps, double eps_u = 0.0) {
+// into_force: FORCE_KICK into the force field F instead of the momenta (F += eps force)
+int stout_force_impl(gf_ctx* c, double coeff, int mode, double eps, double eps_u = 0.0, bool into_force = false) {
     TRY(stout_forward_all(c));
     // the thin-link level's stencil kernel forms the force from dS/dU_bare
-    return stout_backward_all(c, mode, mode == FORCE_STORE ? (void*)c->F : (void*)c->P, eps, eps_u, true, coeff);
+    void* target = (mode == FORCE_STORE || into_force) ? (void*)c->F : (void*)c->P;
+    return stout_backward_all(c, mode, target, eps, eps_u, true, coeff);
 }
 
 int kick(gf_ctx* c, double coeff, double eps) {
@@ -1009,6 +1011,16 @@ int gf_stout_force_kick(gf_ctx* c, double coeff, double eps) {
     CTXCHK(c);
     if (!std::isfinite(eps)) return fail(c, GF_EINVAL, "the momentum step size must be finite; got %g", eps);
     return kick(c, coeff, eps);
+}
+int gf_stout_force_add(gf_ctx* c, double coeff) {
+    CTXCHK(c);
+    if (!std::isfinite(coeff)) return fail(c, GF_EINVAL, "the action coefficient must be finite; got %g", coeff);
+    if (c->nlev > 0) return stout_force_impl(c, coeff, FORCE_KICK, 1.0, 0.0, true);
+    return dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_force<T>(FORCE_KICK, c->geo, (const T*)c->U, (T*)c->F, nullptr, coeff, 1.0, 0.0, c->stream));
+        return GF_OK;
+    });
 }
 
 int gf_round_to_float(gf_ctx* c, int fields) {

@@ -13,8 +13,9 @@ backends do:
   * methods of the reference's functions for `Vector`s of these fields, with the reference's
     signatures: calculate_Plaquette / measure_plaquette, evaluate_GaugeAction, calc_dSdUμ!,
     the MD provider trio md_action_workspace / md_potential / md_force!
-    (molecular_dynamics.jl:13-43, 243-267), md_hamiltonian, update_gaugefields!,
-    update_momenta!, md_trajectory! (:494-730), initialize_TA_Gaugefields /
+    (molecular_dynamics.jl:13-43, 243-267) also for an MDActionSet and an MDForceGroup
+    (:45-236; the reference's SextonWeingarten step then runs unchanged), md_hamiltonian,
+    update_gaugefields!, update_momenta!, md_trajectory! (:494-730), initialize_TA_Gaugefields /
     gauss_distribution! / gaussian_momenta!, flow! (gradientflow.jl:171-238), calc_smearedU /
     back_prop (Abstractsmearing.jl:237-411), forward! / layer_pullback! of STOUT_Layer
     (stout_fast.jl:223-274), heatbath! / overrelaxation!, topological_charge,
@@ -36,6 +37,7 @@ module GaugefieldsHIP
 using LinearAlgebra
 import Gaugefields
 import Gaugefields: AbstractGaugeBackend, GaugeAction, MDDriver, QPQ, PQP, CovNeuralnet, STOUT_Layer,
+    MDActionSet, MDForceGroup,
     Gradientflow, Heatbath, ILDG
 import Gaugefields: gauge_configuration, measure_plaquette, gaussian_momenta!, gauge_momenta,
     md_action_workspace, md_potential, md_force!, md_hamiltonian, update_gaugefields!, update_momenta!,
@@ -509,12 +511,16 @@ function _momenta_dot(c::HIPContext)
     return s[]
 end
 
-"""md_potential(action, U, workspace) = -real(evaluate_GaugeAction(action, U)) / NC, on the smeared
-links for a StoutGaugeAction (Hamiltonian minus p*p/2 of the same context)."""
-function md_potential(action::Union{GaugeAction,StoutGaugeAction}, U::HIPLinks, workspace)
-    c = _context(U)
+# the potential of one provider: the Hamiltonian minus p*p/2 of the same context
+function _provider_potential(c::HIPContext, action)
     coeff = _configure!(c, action)
     return _hamiltonian(c, coeff) - _momenta_dot(c) / 2
+end
+
+"""md_potential(action, U, workspace) = -real(evaluate_GaugeAction(action, U)) / NC, on the smeared
+links for a StoutGaugeAction."""
+function md_potential(action::Union{GaugeAction,StoutGaugeAction}, U::HIPLinks, workspace)
+    return _provider_potential(_context(U), action)
 end
 
 """md_force!(force, action, U, workspace): F = -(1/3) TA(U dS/dU) (through the stout chain for a
@@ -529,9 +535,68 @@ function md_force!(force::HIPMomenta, action::Union{GaugeAction,StoutGaugeAction
     return nothing
 end
 
+# ---- several time scales: an MDActionSet's members (GaugeAction / StoutGaugeAction providers)
+# one after another on the device (molecular_dynamics.jl:45-236); the reference's own
+# md_step!(::SextonWeingarten, ...) (:636-700) drives them through update_momenta!(..., group)
+
+_members(a) = (a,)
+_members(actions::MDActionSet) = values(actions.terms)
+_members(actions::MDActionSet, ::MDForceGroup{Names}) where {Names} = map(n -> getproperty(actions.terms, n), Names)
+
+md_action_workspace(actions::MDActionSet, U::HIPLinks) = nothing
+
+"""md_potential of an MDActionSet (molecular_dynamics.jl:111-126): the sum of the members'."""
+function md_potential(actions::MDActionSet, U::HIPLinks, workspace)
+    c = _context(U)
+    return sum(a -> _provider_potential(c, a), _members(actions))
+end
+
+# the first member's force written, every further one added (_md_write_named_forces!,
+# molecular_dynamics.jl:128-183)
+function _set_force!(force::HIPMomenta, U::HIPLinks, members)
+    length(force) == length(U) || throw(ArgumentError("force and U must have the same number of directions"))
+    c = _context(U)
+    all(f -> f.ctx === c && f.slot === :force, force) ||
+        throw(ArgumentError("the HIP force is the context's force field: pass force_field(U)"))
+    for (k, a) in enumerate(members)
+        coeff = _configure!(c, a)
+        if k == 1
+            _check(c, ccall((:gf_stout_force, libgfhip), Cint, (Ptr{Cvoid}, Cdouble), _ctx(c), coeff))
+        else
+            _check(c, ccall((:gf_stout_force_add, libgfhip), Cint, (Ptr{Cvoid}, Cdouble), _ctx(c), coeff))
+        end
+    end
+    return nothing
+end
+
+"""md_force!(force, actions::MDActionSet, U, workspace) (molecular_dynamics.jl:201-214): the sum
+of the members' forces in the context's force field."""
+function md_force!(force::HIPMomenta, actions::MDActionSet, U::HIPLinks, workspace)
+    return _set_force!(force, U, _members(actions))
+end
+
+"""md_force!(force, actions, U, workspace, group) (molecular_dynamics.jl:216-236): only the
+members the MDForceGroup names."""
+function md_force!(force::HIPMomenta, actions::MDActionSet, U::HIPLinks, workspace, group::MDForceGroup)
+    return _set_force!(force, U, _members(actions, group))
+end
+
+# P += step F_m for each member m: force and kick in one library call each
+function _kick!(c::HIPContext, members, step_size)
+    isfinite(step_size) || throw(ArgumentError("the momentum step size must be finite; got $step_size"))
+    for a in members
+        coeff = _configure!(c, a)
+        _check(c, ccall((:gf_stout_force_kick, libgfhip), Cint, (Ptr{Cvoid}, Cdouble, Cdouble),
+                        _ctx(c), coeff, Float64(step_size)))
+    end
+    return nothing
+end
+
 """md_hamiltonian(U, p, driver) (molecular_dynamics.jl:494-505): md_potential + p*p/2."""
 function md_hamiltonian(U::HIPLinks, P::HIPMomenta, driver::MDDriver)
     c = _same(U, P)
+    driver.action isa MDActionSet && return sum(a -> _provider_potential(c, a), _members(driver.action)) +
+                                            _momenta_dot(c) / 2
     return _hamiltonian(c, _configure!(c, driver.action))
 end
 
@@ -544,13 +609,20 @@ function update_gaugefields!(U::HIPLinks, P::HIPMomenta, step_size, driver::MDDr
 end
 
 """update_momenta!(P, U, step_size, driver) (molecular_dynamics.jl:539-551): P ← P + step F(U),
-force and kick in one kernel (through the stout chain for a StoutGaugeAction)."""
+force and kick in one kernel (through the stout chain for a StoutGaugeAction; member by member
+for an MDActionSet)."""
 function update_momenta!(P::HIPMomenta, U::HIPLinks, step_size, driver::MDDriver)
-    c = _same(U, P)
-    isfinite(step_size) || throw(ArgumentError("the momentum step size must be finite; got $step_size"))
-    coeff = _configure!(c, driver.action)
-    _check(c, ccall((:gf_stout_force_kick, libgfhip), Cint, (Ptr{Cvoid}, Cdouble, Cdouble),
-                    _ctx(c), coeff, Float64(step_size)))
+    _kick!(_same(U, P), _members(driver.action), step_size)
+    return P
+end
+
+"""update_momenta!(P, U, step_size, driver, group) (molecular_dynamics.jl:553-583): the kick of the
+members of the driver's MDActionSet that the MDForceGroup names (Sexton-Weingarten's slow and
+fast kicks)."""
+function update_momenta!(P::HIPMomenta, U::HIPLinks, step_size, driver::MDDriver, group::MDForceGroup)
+    driver.action isa MDActionSet ||
+        throw(ArgumentError("a force group selects members of an MDActionSet; got $(typeof(driver.action))"))
+    _kick!(_same(U, P), _members(driver.action, group), step_size)
     return P
 end
 
@@ -569,13 +641,13 @@ _integrator_code(::PQP) = (GF_PQP, OMELYAN_LAMBDA)
 _integrator_code(i::OmelyanPQP) = (GF_OMELYAN2MN, i.λ)
 
 """md_trajectory!(U, P, driver; diagnostics=true, fused=false) (molecular_dynamics.jl:712-730).
-QPQ, PQP and OmelyanPQP run as one library call per trajectory (`fused=true` merges adjacent
+QPQ, PQP and OmelyanPQP of a single action run as one library call per trajectory (`fused=true` merges adjacent
 half steps into single force + kick + exp kernels); other integrators step through md_step!."""
 function md_trajectory!(U::HIPLinks, P::HIPMomenta, driver::MDDriver; diagnostics::Bool=true,
                         fused::Bool=false)
     c = _same(U, P)
     integ = driver.integrator
-    if integ isa Union{QPQ,PQP,OmelyanPQP}
+    if integ isa Union{QPQ,PQP,OmelyanPQP} && !(driver.action isa MDActionSet)
         coeff = _configure!(c, driver.action)
         code, λ = _integrator_code(integ)
         H = zeros(Float64, 3)

@@ -221,6 +221,10 @@ class Lattice:
     def force(self, coeff):
         self._c(self._L.gf_force(self._ctx, ctypes.c_double(coeff)))
 
+    def force_add(self, coeff):
+        """force field F += F(coeff) (on the smeared links when stout levels are configured)"""
+        self._c(self._L.gf_stout_force_add(self._ctx, ctypes.c_double(coeff)))
+
     def force_kick(self, coeff, eps):
         self._c(self._L.gf_force_kick(self._ctx, ctypes.c_double(coeff), ctypes.c_double(eps)))
 

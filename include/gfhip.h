@@ -189,6 +189,11 @@ int gf_stout_backward(gf_ctx *ctx, const double *dout_ref, double *din_ref);
  * links, back_prop, F = -(1/3) TA(U dSdU_bare) */
 int gf_stout_force(gf_ctx *ctx, double coeff);
 int gf_stout_force_kick(gf_ctx *ctx, double coeff, double eps);
+/* the force of one more member of an MD action set added to the force field: F += F(coeff), on
+ * the smeared links when levels are configured (md_force! of an MDActionSet: the first member's
+ * force written, every further one into a work field and added, _md_add_named_forces!,
+ * molecular_dynamics.jl:128-160) */
+int gf_stout_force_add(gf_ctx *ctx, double coeff);
 
 /* ---- ILDG / LIME configuration files (host only, no context) ----------------------- */
 /* ILDG(filename) header parse (ildg_format.jl:686, read_header :1049-1053): extents lx..lt

@@ -184,6 +184,21 @@ def jl_md_force(R, c, coeff):
     _ok(R.gf_stout_force(c.ptr, coeff))
 
 
+def jl_md_potential_set(R, c, coeffs):  # md_potential(::MDActionSet) of thin plaquette members
+    return sum(jl_md_potential(R, c, k) for k in coeffs)
+
+
+def jl_md_force_set(R, c, coeffs):  # md_force!(force, ::MDActionSet, U, ws[, group]), thin members
+    for k, coeff in enumerate(coeffs):
+        _ok(R.gf_stout_config(c.ptr, 0, 0.0, AUTO))
+        _ok(R.gf_stout_force(c.ptr, coeff) if k == 0 else R.gf_stout_force_add(c.ptr, coeff))
+
+
+def jl_update_momenta_set(R, c, coeffs, eps):  # update_momenta!(P, U, eps, driver, group)
+    for coeff in coeffs:
+        jl_update_momenta(R, c, coeff, eps)
+
+
 def jl_md_hamiltonian(R, c, coeff):
     _ok(R.gf_stout_config(c.ptr, 0, 0.0, AUTO))
     H = ctypes.c_double(0)
@@ -338,11 +353,15 @@ REPLAYS = [
     ("function substitute_U!(U::HIPLinks, Uref", jl_substitute_links, (ZL,), False),
     ("Base.Array(U::HIPLinks)", jl_Array_links, (), True),
     ("function substitute_U!(Uhost::AbstractVector", jl_Array_links, (), True),
-    ("function md_potential(", jl_md_potential, (3.0,), True),
-    ("function md_force!(", jl_md_force, (3.0,), True),
-    ("function md_hamiltonian(", jl_md_hamiltonian, (3.0,), True),
+    ("function md_potential(action::Union", jl_md_potential, (3.0,), True),
+    ("function md_potential(actions::MDActionSet", jl_md_potential_set, ((3.0,),), False),
+    ("function md_force!(force::HIPMomenta, action::Union", jl_md_force, (3.0,), True),
+    ("function md_force!(force::HIPMomenta, actions::MDActionSet, U::HIPLinks, workspace)", jl_md_force_set, ((3.0,),), False),
+    ("workspace, group::MDForceGroup)", jl_md_force_set, ((1.5,),), False),
+    ("function md_hamiltonian(", jl_md_hamiltonian, (3.0,), False),
     ("function update_gaugefields!(", jl_update_gaugefields, (0.02,), True),
-    ("function update_momenta!(", jl_update_momenta, (3.0, 0.02), True),
+    ("update_momenta!(P::HIPMomenta, U::HIPLinks, step_size, driver::MDDriver)", jl_update_momenta, (3.0, 0.02), True),
+    ("driver::MDDriver, group::MDForceGroup)", jl_update_momenta_set, ((0.75,), 0.02), True),
     ("function md_step!(integ::OmelyanPQP", jl_md_step_omelyan, (3.0, 0.1), True),
     ("function md_trajectory!(", jl_md_trajectory, (3.0, 0, LAM, 4, 0.4), False),
     ("function flow!(U::HIPLinks, g::Gradientflow)", jl_flow, (0.01, 3), True),
@@ -461,6 +480,18 @@ def test_replays_against_the_oracle(gf, oracle, tmp_path):
         jl_md_force(R, c, coeff)
         F = oracle.force(dims, U, coeff)
         assert np.abs(jl_Array_momenta(R, c, "force") - F).max() < 1e-12 * np.abs(F).max()
+        # an MDActionSet of two plaquette members: potential, force and a group's kick sum theirs
+        Vs = oracle.potential(dims, U, coeff) + oracle.potential(dims, U, -0.5)
+        assert abs(jl_md_potential_set(R, c, (coeff, -0.5)) - Vs) < 1e-12 * abs(Vs)
+        jl_md_force_set(R, c, (coeff, -0.5))
+        Fs = F + oracle.force(dims, U, -0.5)
+        assert np.abs(jl_Array_momenta(R, c, "force") - Fs).max() < 1e-12 * np.abs(Fs).max()
+        jl_update_momenta_set(R, c, (0.75, 0.25), 0.02)
+        Pk = np.ascontiguousarray(P.copy())
+        oracle.kick(dims, Pk, oracle.force(dims, U, 0.75), 0.02)
+        oracle.kick(dims, Pk, oracle.force(dims, U, 0.25), 0.02)
+        assert np.abs(jl_Array_momenta(R, c, "momenta") - Pk).max() < 1e-12
+        jl_substitute_momenta(R, c, P)
         # one Omelyan step through the per-call methods = the oracle's 2MN trajectory of one step
         jl_backup(R, c)
         jl_md_step_omelyan(R, c, coeff, 0.1)
