@@ -47,6 +47,7 @@ SIGNATURES = {
     "gf_plaquette": (ctypes.c_int, [_CTX, _D]),
     "gf_plaquette_links": (ctypes.c_int, [ctypes.c_void_p, _I, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "gf_action": (ctypes.c_int, [_CTX, ctypes.c_double, _D]),
+    "gf_action_untraced": (ctypes.c_int, [_CTX, ctypes.c_double, _D]),
     "gf_momenta_dot": (ctypes.c_int, [_CTX, _D]),
     "gf_hamiltonian": (ctypes.c_int, [_CTX, ctypes.c_double, _D]),
     "gf_dsdu": (ctypes.c_int, [_CTX, ctypes.c_double, _D]),

@@ -281,6 +281,13 @@ def evaluate_GaugeAction(action, U):
     return U.lattice.action(action.coefficient)
 
 
+def evaluate_GaugeAction_untraced(action, U):
+    """the untraced loop sum of the action at every site (evaluate_GaugeAction_untraced,
+    GaugeActions.jl:144-213): coeff sum_{mu<nu} (P_munu + P_munu^dag), (V, 3, 3) complex [i, j];
+    its trace summed over sites is evaluate_GaugeAction"""
+    return U.lattice.action_untraced(action.coefficient)
+
+
 def calc_dSdUmu(action, mu, U):
     """dS/dU_mu (3x3 per site) for mu in 1..4, as (V, 3, 3) complex128 [i, j]"""
     if mu not in (1, 2, 3, 4):

@@ -702,6 +702,19 @@ int gf_action(gf_ctx* c, double coeff, double* value) {
     *value = 2.0 * coeff * s;
     return GF_OK;
 }
+int gf_action_untraced(gf_ctx* c, double coeff, double* ref) {
+    CTXCHK(c);
+    if (!ref) return fail(c, GF_EINVAL, "the output array must not be null");
+    TRY(need_staging(c));
+    TRY(dispatch(c, [&](auto* tag) -> int {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        HIPCHK(c, launch_action_untraced<T>(c->geo, (const T*)c->U, coeff, c->staging, c->stream));
+        return GF_OK;
+    }));
+    HIPCHK(c, hipMemcpyAsync(ref, c->staging, (size_t)18 * c->V * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return GF_OK;
+}
 int gf_momenta_dot(gf_ctx* c, double* value) {
     CTXCHK(c);
     if (!value) return GF_EINVAL;

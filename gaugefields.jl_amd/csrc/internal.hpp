@@ -32,6 +32,8 @@ hipError_t launch_sum_partials(const double* partial, int n, double* out, hipStr
 template <typename T>
 hipError_t launch_plaquette(const Geo& g, const T* U, double* partial, double* out, hipStream_t st);
 template <typename T>
+hipError_t launch_action_untraced(const Geo& g, const T* U, double coeff, double* ref, hipStream_t st);
+template <typename T>
 hipError_t launch_momenta_dot(const Geo& g, const T* P, double* partial, double* out, hipStream_t st);
 template <typename T>
 hipError_t launch_dsdu(const Geo& g, const T* U, double coeff, T* D, hipStream_t st);

@@ -88,6 +88,46 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     block_sum_store<kBlock>(v, partial);
 }
 
+// untraced plaquette action (evaluate_GaugeAction_untraced!, GaugeActions.jl:190-213, for the loop
+// set plaquette u plaquette^dag): coeff sum_{mu<nu} (P_munu(x) + P_munu(x)^dag) per site, written
+// in the host layout (3x3 complex, column-major, site-major: element (i, j) of site s at
+// ref[18 s + 2 (i + 3 j)]).  Products in FP64 for both storage precisions.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_action_untraced(Geo g, const T* __restrict__ U, double coeff,
+                                                            double* __restrict__ ref) {
+    const int s = site_index();
+    if (s >= g.V) return;
+    int x[4];
+    decode(g, s, x);
+    const unsigned V = g.V;
+    auto ld = [&](int d, int t) { return full_of(r6_f64(ldr(U, V, d, (unsigned)t))); };
+    M3<double> m = zero3<double>();
+#pragma unroll 1
+    for (int pl = 0; pl < 6; pl++) {
+        const int mu = pl < 3 ? 0 : (pl < 5 ? 1 : 2);
+        const int nu = pl < 3 ? pl + 1 : (pl < 5 ? pl - 1 : 3);
+        const M3<double> a = mul(ld(mu, s), ld(nu, s + dlt(g, x, mu, 1)));   // U_mu(x) U_nu(x+mu)
+        const M3<double> b = mul(ld(nu, s), ld(mu, s + dlt(g, x, nu, 1)));   // U_nu(x) U_mu(x+nu)
+        const M3<double> P = mul_nd(a, b);                                   // a b^dag
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                m.m[3 * i + j].r += P.m[3 * i + j].r + P.m[3 * j + i].r;
+                m.m[3 * i + j].i += P.m[3 * i + j].i - P.m[3 * j + i].i;
+            }
+    }
+    double* r = ref + (size_t)s * 18;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int c = i + 3 * j;
+            r[2 * c] = coeff * m.m[3 * i + j].r;
+            r[2 * c + 1] = coeff * m.m[3 * i + j].i;
+        }
+}
+
 // sum of p_a^2 over all links (p*p, TA_gaugefields_4D_serial.jl:106-129)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_momenta_dot(Geo g, const T* __restrict__ P, double* __restrict__ partial) {
@@ -391,6 +431,11 @@ hipError_t launch_plaquette(const Geo& g, const T* U, double* partial, double* o
     return hipGetLastError();
 }
 template <typename T>
+hipError_t launch_action_untraced(const Geo& g, const T* U, double coeff, double* ref, hipStream_t st) {
+    k_action_untraced<T><<<grid_for(g.V), kBlock, 0, st>>>(g, U, coeff, ref);
+    return hipGetLastError();
+}
+template <typename T>
 hipError_t launch_momenta_dot(const Geo& g, const T* P, double* partial, double* out, hipStream_t st) {
     const int nb = grid_sites_mu(g.V);
     k_momenta_dot<T><<<nb, kBlock, 0, st>>>(g, P, partial);
@@ -491,6 +536,7 @@ hipError_t launch_axpby(T* y, const T* x, double a, double b, size_t n, hipStrea
 
 #define GF_INST(T)                                                                                       \
     template hipError_t launch_plaquette<T>(const Geo&, const T*, double*, double*, hipStream_t);        \
+    template hipError_t launch_action_untraced<T>(const Geo&, const T*, double, double*, hipStream_t);  \
     template hipError_t launch_momenta_dot<T>(const Geo&, const T*, double*, double*, hipStream_t);      \
     template hipError_t launch_dsdu<T>(const Geo&, const T*, double, T*, hipStream_t);                   \
     template hipError_t launch_force<T>(int, const Geo&, const T*, T*, T*, double, double, double,       \

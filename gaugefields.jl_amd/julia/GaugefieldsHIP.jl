@@ -45,7 +45,8 @@ import Gaugefields: gauge_configuration, measure_plaquette, gaussian_momenta!, g
 import Gaugefields.AbstractGaugefields_module: Gaugefields_4D, TA_Gaugefields_4D, calculate_Plaquette,
     substitute_U!, set_wing_U!, clear_U!, initialize_TA_Gaugefields, gauss_distribution!, topological_charge,
     topological_charge_density
-import Gaugefields.GaugeAction_module: evaluate_GaugeAction, calc_dSdUμ!
+import Gaugefields.GaugeAction_module: evaluate_GaugeAction, calc_dSdUμ!, evaluate_GaugeAction_untraced,
+    evaluate_GaugeAction_untraced!
 import Gaugefields.Gradientflow_module: flow!
 import Gaugefields.Abstractsmearing_module: calc_smearedU, back_prop, forward!, layer_pullback!
 import Gaugefields.heatbath_module: heatbath!, overrelaxation!
@@ -398,6 +399,25 @@ end
 
 """calc_dSdUμ!(dSdUμ, action, μ, U) with a host (3, 3, NX, NY, NZ, NT) array as the output
 (the device computes all four directions in one kernel)."""
+"""evaluate_GaugeAction_untraced!(uout, action, U) (GaugeActions.jl:190-213) for the loop set
+plaquette ∪ plaquette': the per-site loop sum coeff Σ_{μ<ν} (P_μν(x) + P_μν(x)†) into a host
+array of the reference's layout (3, 3, NX, NY, NZ, NT)."""
+function evaluate_GaugeAction_untraced!(uout::AbstractArray{ComplexF64,6}, action::GaugeAction, U::HIPLinks)
+    c = _context(U)
+    size(uout) == (3, 3, c.dims...) || throw(DimensionMismatch("uout must be (3, 3, $(join(c.dims, ", ")))"))
+    _check(c, ccall((:gf_action_untraced, libgfhip), Cint, (Ptr{Cvoid}, Cdouble, Ptr{ComplexF64}),
+                    _ctx(c), _plaquette_coefficient(action), uout))
+    return nothing
+end
+
+"""evaluate_GaugeAction_untraced(action, U) (GaugeActions.jl:144-160): the same, allocated."""
+function evaluate_GaugeAction_untraced(action::GaugeAction, U::HIPLinks)
+    c = _context(U)
+    uout = Array{ComplexF64,6}(undef, 3, 3, c.dims...)
+    evaluate_GaugeAction_untraced!(uout, action, U)
+    return uout
+end
+
 function calc_dSdUμ!(dSdUμ::AbstractArray{ComplexF64,6}, action::GaugeAction, μ, U::HIPLinks)
     1 <= μ <= 4 || throw(ArgumentError("μ must be in 1:4; got $μ"))
     coeff = _plaquette_coefficient(action)

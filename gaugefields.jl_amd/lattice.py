@@ -173,6 +173,13 @@ class Lattice:
     def action(self, coeff):
         return self._scalar(self._L.gf_action, ctypes.c_double(coeff))
 
+    def action_untraced(self, coeff):
+        """coeff sum_{mu<nu} (P_munu + P_munu^dag) per site, (V, 3, 3) complex [i, j]"""
+        ref = np.empty((self.V, 3, 3), np.complex128, order="C")
+        self._c(self._L.gf_action_untraced(self._ctx, ctypes.c_double(coeff), _dptr(ref)))
+        # the host layout is column-major per site: element (i, j) at 3 j + i
+        return ref.transpose(0, 2, 1).copy()
+
     def momenta_dot(self):
         return self._scalar(self._L.gf_momenta_dot)
 

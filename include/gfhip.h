@@ -115,6 +115,11 @@ int gf_plaquette_links(const void *links, const int *dims, int precision, void *
 /* real(evaluate_GaugeAction) (GaugeActions.jl:132-142) of push!(action, coeff,
  * plaquette ∪ plaquette^dag): 2*coeff*sum Re tr P.  md_potential = -value/3. */
 int gf_action(gf_ctx *ctx, double coeff, double *value);
+/* evaluate_GaugeAction_untraced! (GaugeActions.jl:190-213) for the loop set plaquette u
+ * plaquette^dag: coeff sum_{mu<nu} (P_munu(x) + P_munu(x)^dag) at every site, into ref as one
+ * 3x3 complex matrix per site in the reference's layout (column-major, sites x fastest; 9 V
+ * complex).  Its trace summed over sites is gf_action's value. */
+int gf_action_untraced(gf_ctx *ctx, double coeff, double *ref);
 /* p*p (TA_gaugefields_4D_MPILattice.jl:147-155): sum of all p_a^2 */
 int gf_momenta_dot(gf_ctx *ctx, double *value);
 /* md_hamiltonian (molecular_dynamics.jl:494-505): md_potential + p*p/2, on the stout-smeared

@@ -53,6 +53,18 @@ def plaquette_field_sum(U, dims):
     return tot
 
 
+def untraced_plaquette_action(U, dims, coeff):
+    """coeff sum_{mu<nu} (P_munu(x) + P_munu(x)^dag) per site as (V, 3, 3): the untraced loop sum
+    of the loop set plaquette u plaquette^dag (every loop evaluated from its starting site x)"""
+    M = field(U, dims)
+    out = 0
+    for mu in range(4):
+        for nu in range(mu + 1, 4):
+            P = M[mu] @ sh(M[nu], mu) @ dg(sh(M[mu], nu)) @ dg(M[nu])
+            out = out + P + dg(P)
+    return coeff * out.reshape(-1, 3, 3)
+
+
 def ta_matrix(M):
     A = 0.5 * (M - dg(M))
     tr = np.trace(A, axis1=-2, axis2=-1)

@@ -136,6 +136,29 @@ def test_potential_force_and_derivative(api, oracle):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dims,precision", [(D4, "f64"), ((4, 2, 6, 4), "f64"), ((8, 8, 8, 8), "f32")])
+def test_untraced_action(api, dims, precision):
+    # evaluate_GaugeAction_untraced (GaugeActions.jl:144-213): the per-site loop sum against a
+    # numpy np.roll restatement, and tr of it summed over sites = evaluate_GaugeAction
+    print(f"[untraced action] {dims} {precision}", flush=True)
+    U = api.gauge_configuration(dims, start="hot", seed=61)
+    if precision == "f32":  # the links as float holds them (row 2 rebuilt in double)
+        W = api.gauge_configuration(dims, eltype=np.complex64)
+        W.from_host(U.to_host())
+        U = W
+    host = U.to_host()
+    a = _action(api, U, 2.5)
+    M = api.evaluate_GaugeAction_untraced(a, U)
+    ref = lt.untraced_plaquette_action(host, dims, 2.5)
+    assert np.abs(M - ref).max() < 1e-12 * np.abs(ref).max()
+    tr = np.trace(M, axis1=1, axis2=2).sum()
+    S = api.evaluate_GaugeAction(a, U)
+    assert abs(tr.imag) < 1e-10 and tr.real == pytest.approx(S, rel=1e-12)
+    # Hermitian by construction (the loop set is closed under conjugation)
+    assert np.abs(M - np.conj(np.swapaxes(M, 1, 2))).max() < 1e-13 * np.abs(M).max()
+
+
+@pytest.mark.gpu
 def test_md_cold_start_and_custom_integrator(api):
     # md_driver.jl:258-282 cold start; :417-464 custom QPQ == built-in QPQ
     U = api.gauge_configuration(D4)

@@ -184,6 +184,12 @@ def jl_md_force(R, c, coeff):
     _ok(R.gf_stout_force(c.ptr, coeff))
 
 
+def jl_evaluate_GaugeAction_untraced(R, c, coeff):  # (3, 3, NX, NY, NZ, NT) column-major
+    out = np.empty(9 * c.V, np.complex128)
+    _ok(R.gf_action_untraced(c.ptr, coeff, _d(out)))
+    return out.reshape(c.V, 3, 3).transpose(0, 2, 1)  # site-major (V, i, j)
+
+
 def jl_md_potential_set(R, c, coeffs):  # md_potential(::MDActionSet) of thin plaquette members
     return sum(jl_md_potential(R, c, k) for k in coeffs)
 
@@ -345,6 +351,8 @@ REPLAYS = [
     ("function measure_plaquette(", jl_measure_plaquette, (), True),
     ("function evaluate_GaugeAction(", jl_evaluate_GaugeAction, (3.0,), True),
     ("function calc_dSdUμ!(", jl_calc_dSdU, (3.0, 2), True),
+    ("function evaluate_GaugeAction_untraced!(", jl_evaluate_GaugeAction_untraced, (3.0,), True),
+    ("function evaluate_GaugeAction_untraced(action", jl_evaluate_GaugeAction_untraced, (3.0,), True),
     ("function gauss_distribution!(", jl_gauss_distribution, (1.0, 5, 1), True),
     ("function Base.:*(", jl_pp, (), True),
     ("function clear_U!(P::HIPMomenta)", jl_clear_momenta, (), True),
@@ -466,6 +474,9 @@ def test_replays_against_the_oracle(gf, oracle, tmp_path):
         assert abs(jl_calculate_Plaquette(R, c) - p) < 1e-12 * abs(p)
         assert abs(jl_measure_plaquette(R, c) - p / (18 * V)) < 1e-12
         assert abs(jl_evaluate_GaugeAction(R, c, coeff) - 2 * coeff * p) < 1e-12 * abs(2 * coeff * p)
+        Mu = jl_evaluate_GaugeAction_untraced(R, c, coeff)
+        Mref = lt.untraced_plaquette_action(U, dims, coeff)
+        assert np.abs(Mu - Mref).max() < 1e-12 * np.abs(Mref).max()
         Dref = oracle.dsdu(dims, U, coeff)
         assert np.abs(jl_calc_dSdU(R, c, coeff, 2) - Dref[1]).max() < 1e-12 * np.abs(Dref).max()
         # momenta: device Philox stream = oracle stream
