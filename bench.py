@@ -494,7 +494,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the lattice would shard by domain decomposition; see DESIGN.md)", file=sys.stderr)
+              "(the lattice would shard by Cartesian domain decomposition with halo exchange)", file=sys.stderr)
         sys.exit(2)
 
     gf = __graft_entry__.load_package()

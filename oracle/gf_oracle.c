@@ -1,7 +1,7 @@
 /*
  * gf_oracle.c -- CPU FP64 restatement of Gaugefields.jl's SU(3) gauge-link hot path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, __graft_entry__.smoke() and the
+ * TEST INFRASTRUCTURE ONLY (the oracle).  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load this library; the HIP product never uses it.
  *
  * Every function names the reference file:line it restates (paths relative to the

@@ -1,7 +1,7 @@
 /*
  * gf_oracle.h -- CPU FP64 restatement of Gaugefields.jl's SU(3) gauge-link hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP library is
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle that the HIP library is
  * compared against.  Only tests/, __graft_entry__.smoke() and the cpu_baseline leg
  * of bench.py may load it.  The product path (libgfhip) never links, loads or calls
  * anything in oracle/.

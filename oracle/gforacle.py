@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/_build/libgforacle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product (libgfhip) never uses it.
+cpu_baseline leg of bench.py, as the oracle.  The product (libgfhip) never uses it.
 
 All arrays are numpy float64 in the reference's Julia layouts:
   links   (4, V, 9) complex128, element (i, j) at i + 3*j (column-major 3x3),
