@@ -493,7 +493,7 @@ def main():
                     help="unconditionally accepted thermalization trajectories before --production")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(the lattice would shard by Cartesian domain decomposition with halo exchange)", file=sys.stderr)
         sys.exit(2)
 
