@@ -8,8 +8,9 @@
 // (samples/measurements/energydensity.jl:32-76).
 //
 // One workgroup = 64 sites x 4 wavefronts; wavefront w evaluates the planes w and w + 4 of the
-// six (mu < nu) and leaves the 8 su(3) coefficients of each F in LDS; wavefront 0 then combines
-// the planes of each site.  With F = sum_a c_a i lambda_a / 2, tr(F F') = -(1/2) c . c', so
+// six (mu < nu) (with both clover and rectangle sets: the rectangles of planes 2w, 2w + 1 for
+// w < 3, every clover for w = 3) and leaves the 8 su(3) coefficients of each F in LDS;
+// wavefront 0 then combines the planes of each site.  With F = sum_a c_a i lambda_a / 2, tr(F F') = -(1/2) c . c', so
 //   sum_{mu nu rho sigma} eps tr(F_munu F_rhosigma) = -4 (c01.c23 - c02.c13 + c03.c12)
 // and the density -norm Re(...) / (32 pi^2 n_loops^2) = norm (c01.c23 - c02.c13 + c03.c12) /
 // (8 pi^2 n_loops^2); the clover energy sum_{mu != nu} -tr(F F)/2 / 16 = sum_planes |c|^2 / 32.
@@ -95,12 +96,22 @@ __global__ __launch_bounds__(kBlock) void k_fstrength(Geo g, const T* __restrict
     if (valid) {
         int x[4];
         decode(g, sm.s, x);
+        // with clover and rectangle sets (a rectangle loop is 6 links, a clover leaf 4): waves
+        // 0..2 evaluate the rectangles of two planes each and wave 3 the clovers (and
+        // plaquettes) of all six, 96 link steps per wave instead of 128 / 64 (32^4 improved
+        // charge 2.45 -> 2.33 ms: the other workgroups on a CU already fill most of the idle
+        // waves' time); otherwise wave w takes planes w, w + 4
+        constexpr bool SPLIT = (SETS & 2) && (SETS & 4);
+        const int pl0 = SPLIT ? (sm.mu < 3 ? 2 * sm.mu : 0) : sm.mu;
+        const int pl1 = SPLIT ? (sm.mu < 3 ? 2 * sm.mu + 2 : 6) : 6;
+        const int dpl = SPLIT ? 1 : 4;
+        const bool do_cl = !SPLIT || sm.mu == 3, do_rect = !SPLIT || sm.mu < 3;
 #pragma unroll 1
-        for (int pl = sm.mu; pl < 6; pl += 4) {
+        for (int pl = pl0; pl < pl1; pl += dpl) {
             const int mu = pl < 3 ? 0 : (pl < 5 ? 1 : 2);
             const int nu = pl < 3 ? pl + 1 : (pl < 5 ? pl - 1 : 3);
             double c[8];
-            if (SETS & 3) {
+            if ((SETS & 3) && do_cl) {
                 // clover leaf 0 is the plaquette
                 M3<double> acc = walk(g, U, V, x, mu, nu, kClover[0]);
                 if (SETS & 1) {
@@ -123,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_fstrength(Geo g, const T* __restrict
                     for (int a = 0; a < 8; a++) cf[IC][pl][a][lane] = c[a];
                 }
             }
-            if (SETS & 4) {
+            if ((SETS & 4) && do_rect) {
                 M3<double> acc = zero3<double>();
 #pragma unroll 1
                 for (int k = 0; k < 8; k++) {
