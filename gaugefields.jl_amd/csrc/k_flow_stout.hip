@@ -437,8 +437,9 @@ __device__ __forceinline__ void back_local_stored(const T* __restrict__ Uin, con
 }
 template <typename T>
 constexpr int kChainLds = (int)(2 * 4 * 9 * 64 * sizeof(T2<T>));  // dst + cst; >= ul + hl
+// FP32: three waves per SIMD (at most 168 VGPRs); FP64 holds two (its LDS allows two workgroups per CU)
 template <typename T, int ST>
-__global__ __launch_bounds__(kBlock) void k_stout_back_chain(Geo g, const T* __restrict__ U, const T* __restrict__ Lc,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) void k_stout_back_chain(Geo g, const T* __restrict__ U, const T* __restrict__ Lc,
                                                              T* __restrict__ Din, double rho,
                                                              const T* __restrict__ Uprev, const T* __restrict__ Cprev,
                                                              T* __restrict__ Lout) {

@@ -247,8 +247,17 @@ __device__ __forceinline__ void complete_row2(M3<T>& m) {
     for (int k = 0; k < 3; k++) {
         const int a = (k + 1) % 3, b = (k + 2) % 3;
         const cx<T> p0 = m.m[a], p1 = m.m[b], q0 = m.m[3 + a], q1 = m.m[3 + b];
-        m.m[6 + k].r = (p0.r * q1.r - p0.i * q1.i) - (p1.r * q0.r - p1.i * q0.i);
-        m.m[6 + k].i = -((p0.r * q1.i + p0.i * q1.r) - (p1.r * q0.i + p1.i * q0.r));
+        // one product and three FMAs per part (4 instead of 5 operations)
+        T re = p0.r * q1.r;
+        re = fma(-p0.i, q1.i, re);
+        re = fma(-p1.r, q0.r, re);
+        re = fma(p1.i, q0.i, re);
+        T im = -p0.r * q1.i;
+        im = fma(-p0.i, q1.r, im);
+        im = fma(p1.r, q0.i, im);
+        im = fma(p1.i, q0.r, im);
+        m.m[6 + k].r = re;
+        m.m[6 + k].i = im;
     }
 }
 template <typename T>
