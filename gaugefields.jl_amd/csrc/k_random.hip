@@ -288,7 +288,7 @@ __device__ __forceinline__ int colour_site(const Geo& g, int h, int par, int x[4
 // (SUN_overrelaxation_rng!, heatbathmodule.jl:1295-1321, subgroup i drawn from block i of tag 6:
 // n = floor(2 w0 / 2^32), k = n + 1 + floor((2 - n) w1 / 2^32)).
 template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_heatbath(Geo g, T* __restrict__ U, int mu, int par, double beta,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 3))) void k_heatbath(Geo g, T* __restrict__ U, int mu, int par, double beta,
                                                      unsigned long long seed, unsigned long long sweep, int itmax,
                                                      double* __restrict__ partial) {
     const int h = heatbath_half_site(g);
@@ -342,10 +342,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 }
 
 // Dynamic LDS (unused) of an FP64 heatbath launch: 56 KB per workgroup leaves room for two
-// workgroups per CU instead of the three the 168 VGPRs allow.  A heatbath thread owns one site of
-// one colour, so the sites in flight per XCD are eight times the stencil kernels'; with a third
-// fewer of them the links their staples share stay in L2 between uses (32^4: sweep 1.81 -> 1.52 ms,
-// overrelaxation 1.61 -> 1.42 ms).  FP32 links are half the bytes and keep three workgroups.
+// workgroups per CU instead of three.  A heatbath thread owns one site of one colour, so the sites
+// in flight per XCD are eight times the stencil kernels'; with a third fewer of them the links their
+// staples share stay in L2 between uses (32^4: sweep 1.81 -> 1.52 ms, overrelaxation 1.61 -> 1.42
+// ms).  Two waves per SIMD also let the FP64 kernel use up to 256 VGPRs (its amdgpu_waves_per_eu),
+// which removes the spills a 168-VGPR budget forced (sweep 1.48 -> 1.37 ms, overrelaxation 1.52 ->
+// 1.42 ms, same box).  FP32 links are half the bytes and keep three workgroups.
 constexpr size_t kHeatbathOccLds = 56 * 1024;
 
 template <typename T>
