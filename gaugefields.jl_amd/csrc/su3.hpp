@@ -36,6 +36,9 @@ __device__ __forceinline__ T fma1(double a, T x, T y) {
     return (T)fma(a, (double)x, (double)y);
 }
 
+// true when pred holds on every active lane of the wave (a wave-uniform branch condition)
+__device__ __forceinline__ bool wave_all(bool pred) { return __builtin_amdgcn_ballot_w64(!pred) == 0ull; }
+
 // ---------------------------------------------------------------------------
 // complex helpers
 // ---------------------------------------------------------------------------
@@ -220,7 +223,15 @@ __device__ __forceinline__ void reunitarize_f64(M3<double>& u) {
         w1r += r2[c].r * r1[c].r + r2[c].i * r1[c].i;  // row2 . conj(row1)
         w1i += r2[c].i * r1[c].r - r2[c].r * r1[c].i;
     }
-    const double inv2 = 1.0 / w2;
+    // Rows that are unit to within |d| <= 2^-14 (every updated link: an exactly unitary factor
+    // times a stored link) take 1/(1+d) and 1/sqrt(1+d) from their series to d^4 (truncation
+    // below 2e-21); a wave with any other row redoes them with the division and square roots.
+    // 1/(1+d) = 1 - d + d^2 - d^3 + d^4,  1/sqrt(1+d) = 1 - d/2 + 3d^2/8 - 5d^3/16 + 35d^4/128
+    auto rcp1 = [](double d) { return fma(fma(fma(d - 1.0, d, 1.0), d, -1.0), d, 1.0); };
+    auto rsq1 = [](double d) { return fma(fma(fma(fma(d, 35.0 / 128.0, -5.0 / 16.0), d, 3.0 / 8.0), d, -0.5), d, 1.0); };
+    const bool exact = !wave_all(fabs(w2 - 1.0) <= 0x1p-14);
+    double inv2 = rcp1(w2 - 1.0);
+    if (exact) inv2 = 1.0 / w2;
     w1r = -w1r * inv2;
     w1i = -w1i * inv2;
     double w3 = 0.0;
@@ -232,7 +243,11 @@ __device__ __forceinline__ void reunitarize_f64(M3<double>& u) {
         r2[c].i = xi;
         w3 += xr * xr + xi * xi;
     }
-    const double n1 = 1.0 / sqrt(w2), n2 = 1.0 / sqrt(w3);
+    double n1 = rsq1(w2 - 1.0), n2 = rsq1(w3 - 1.0);
+    if (exact || !wave_all(fabs(w3 - 1.0) <= 0x1p-14)) {
+        n1 = 1.0 / sqrt(w2);
+        n2 = 1.0 / sqrt(w3);
+    }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         r1[c].r *= n1;
@@ -622,9 +637,8 @@ __device__ __forceinline__ void ch_coeffs(double c0, double c1, dc f[3], dc b1[3
 // more (checked against scipy's expm and expm_frechet: 4e-16 and 1.4e-15).  Straight-line FMAs
 // instead of acos, three sincos, square roots and divisions; no special case at small c1.
 #define GF_CH_SERIES_C1 0.75
-template <bool WANT_B>
+template <bool WANT_B, int N = WANT_B ? 20 : 18>
 __device__ __forceinline__ void ch_coeffs_series(double c0, double c1, dc f[3], dc b1[3], dc b2[3]) {
-    constexpr int N = WANT_B ? 20 : 18;
     double a = 1.0, b = 0.0, c = 0.0;
     double a0 = 0.0, bb0 = 0.0, cc0 = 0.0, a1 = 0.0, bb1 = 0.0, cc1 = 0.0;  // d/dc0, d/dc1
     double fr[3] = {0.0, 0.0, 0.0}, fi[3] = {0.0, 0.0, 0.0};
@@ -670,10 +684,23 @@ __device__ __forceinline__ void ch_coeffs_series(double c0, double c1, dc f[3], 
         }
     }
 }
+// Shorter series for the exponential alone when every lane of the wave has small eigenvalues:
+// the matrix tail after N terms is at most e^r r^N / N! for eigenvalues of modulus <= r, with
+// r^2 <= 4 c1 / 3.  r <= 0.1 (c1 <= 0.0075): 11 terms, tail <= 2.8e-19; r <= 0.25
+// (c1 <= 0.046875): 13 terms, tail <= 3.1e-18 (MD steps and flow stages are in these ranges).
+// The derivative coefficients keep the full length: their basis 1, H, H^2 does not shrink the
+// tail of the individual coefficients.
+#define GF_CH_SHORT1_C1 0.0075
+#define GF_CH_SHORT2_C1 0.046875
+
 // f_j (and b_1j, b_2j) of exp(iH): the series for c1 <= 3/4, the closed form beyond
 template <bool WANT_B>
 __device__ __forceinline__ void exp_coeffs(double c0, double c1, dc f[3], dc b1[3], dc b2[3]) {
-    if (c1 <= GF_CH_SERIES_C1)
+    if (!WANT_B && wave_all(c1 <= GF_CH_SHORT1_C1))
+        ch_coeffs_series<false, 11>(c0, c1, f, b1, b2);
+    else if (!WANT_B && wave_all(c1 <= GF_CH_SHORT2_C1))
+        ch_coeffs_series<false, 13>(c0, c1, f, b1, b2);
+    else if (c1 <= GF_CH_SERIES_C1)
         ch_coeffs_series<WANT_B>(c0, c1, f, b1, b2);
     else
         ch_coeffs<WANT_B>(c0, c1, f, b1, b2);
