@@ -182,9 +182,10 @@ def test_fused_md_update(Lat, oracle):
 
 def test_exp_large_and_tiny_momenta(Lat, oracle):
     L = Lat()
-    # c1 ~ 2 scale^2 per link: 3.0 the closed form, 0.45 both sides of c1 = 3/4 within a wave,
-    # 0.2 and below the series
-    for scale in (3.0, 0.45, 0.2, 1e-9, 0.0):
+    # c1 ~ 2 scale^2 per link (~5 scale^2 the largest of a wave): 3.0 the closed form, 0.45 both
+    # sides of c1 = 3/4 within a wave, 0.2 the 18-term series, 0.09 waves on both sides of the
+    # 13-term limit (c1 = 0.046875), 0.08 the 13-term series, 0.03, 1e-9 and 0 the 11-term one
+    for scale in (3.0, 0.45, 0.2, 0.09, 0.08, 0.03, 1e-9, 0.0):
         P = scale * gen.gaussian_momenta(D4, 17)
         L.set_cold()
         L.upload_momenta(P)
