@@ -237,7 +237,8 @@ def stout_hmc_thermalized(gf, dims=DIMS, ntherm=6, npairs=8):
     momenta (both exactly representable in FP32) in FP64 and in FP32 bulk: s/traj (the faster
     of two runs of the first pair), dH agreement per pair (the FP32-bulk deviation is a sum of
     storage roundings along the trajectory, so it is reported over several momentum draws:
-    each, RMS, median and max), plaquette agreement, FP32 reversibility"""
+    each, RMS, median and max), plaquette agreement, FP32 reversibility.  The same trajectories
+    also run in the FP32-force form ("f32force": FP64 links and momenta, FP32 forces)."""
     gen = gf.generators
     L64 = make_lattice(gf, dims, "f64", seed_u=777, seed_p=778)
     L64.stout_config(3, 0.12, "store")
@@ -251,12 +252,14 @@ def stout_hmc_thermalized(gf, dims=DIMS, ntherm=6, npairs=8):
     # both forms start from the identical configuration: the thermalized links as FP32 holds
     # them (rows 0 and 1 rounded to float, row 2 rebuilt) and float-representable momenta
     U0 = L32.download_links()
-    times = {"fp64": [], "fp32": []}
+    L3 = gf.Lattice(dims, precision="f32force")
+    L3.stout_config(3, 0.12, "store")
+    times = {"fp64": [], "fp32": [], "f32force": []}
     pairs, plaq = [], {}
     for k in range(npairs):
         P0 = gen.gaussian_momenta(dims, 9999 + k).astype(np.float32).astype(np.float64)
         dH = {}
-        for tag, L in (("fp64", L64), ("fp32", L32)):
+        for tag, L in (("fp64", L64), ("fp32", L32), ("f32force", L3)):
             for rep in range(2 if k == 0 else 1):
                 L.upload_links(U0)
                 L.upload_momenta(P0)
@@ -269,7 +272,8 @@ def stout_hmc_thermalized(gf, dims=DIMS, ntherm=6, npairs=8):
                 dH[tag] = H[2]
             if k == 0:
                 plaq[tag] = L.plaquette()
-        pairs.append({"dH_fp64": dH["fp64"], "dH_fp32": dH["fp32"], "abs_diff": abs(dH["fp64"] - dH["fp32"])})
+        pairs.append({"dH_fp64": dH["fp64"], "dH_fp32": dH["fp32"], "abs_diff": abs(dH["fp64"] - dH["fp32"]),
+                      "dH_f32force": dH["f32force"], "abs_diff_f32force": abs(dH["fp64"] - dH["f32force"])})
     # FP32 reversibility: the first pair's momenta forward, then back with tau -> -tau
     L32.upload_links(U0)
     L32.upload_momenta(gen.gaussian_momenta(dims, 9999).astype(np.float32).astype(np.float64))
@@ -278,7 +282,9 @@ def stout_hmc_thermalized(gf, dims=DIMS, ntherm=6, npairs=8):
     rev = float(np.abs(L32.download_links() - U0).max())
     L64.close()
     L32.close()
+    L3.close()
     diffs = np.array([p_["abs_diff"] for p_ in pairs])
+    d3 = np.array([p_["abs_diff_f32force"] for p_ in pairs])
     return {
         "lattice": "x".join(map(str, dims)), "levels": 3, "rho": 0.12, "integrator": "2MN", "steps": 16,
         "tau": 1.0, "thermalization_dH": dh_therm,
@@ -290,10 +296,14 @@ def stout_hmc_thermalized(gf, dims=DIMS, ntherm=6, npairs=8):
         "abs_dH_diff_max": float(diffs.max()),
         "plaquette_diff": abs(plaq["fp64"] - plaq["fp32"]),
         "fp32_reversibility_max_abs_dU": rev,
+        "f32force": {"s_per_traj": min(times["f32force"]),
+                     "speedup_over_fp64": min(times["fp64"]) / min(times["f32force"]),
+                     "abs_dH_diff_rms": float(np.sqrt(np.mean(d3 ** 2))), "abs_dH_diff_median": float(np.median(d3)),
+                     "abs_dH_diff_max": float(d3.max()), "plaquette_diff": abs(plaq["fp64"] - plaq["f32force"])},
     }
 
 
-def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232):
+def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232, nf32=16):
     """Which rounding sets the FP32-bulk dH deviation (3-level stout HMC, rho 0.12, 2MN 16 steps,
     tau 1, 32^4).  Thermalization: nforce FP64 trajectories accepted unconditionally from
     omega ~ N(0, 0.3^2) links, then nmetro with Metropolis (acceptance reported).  Each of ndraws
@@ -304,7 +314,10 @@ def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232):
       round_links          FP64, links rounded to float after every update (after the SU(3)
                            projection in double, as the FP32 link updates do);
       round_links_noproj   the same without the projection (links projected only at the start);
-      round_both           round_links and round_momenta.
+      round_both           round_links and round_momenta;
+      fp32_forces          FP64 links and momenta, every force evaluated by an FP32-bulk context
+                           (the whole stout chain in FP32) from the current links rounded to float
+                           (through the host; the first nf32 draws only).
     The FP64 variants step 2MN from the host (kick, update, kick, update, kick per step) with
     gf_round_to_float between the steps; the reference dH is the same unrounded sequence."""
     gen = gf.generators
@@ -331,13 +344,17 @@ def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232):
     L32 = gf.Lattice(dims, precision="f32")
     L32.stout_config(3, 0.12, "store")
 
-    def stepped(rl, rm, project=True):
+    def stepped(rl, rm, project=True, f32force=False):
         H0 = L64.hamiltonian(COEFF)
         eps = 1.0 / 16
         for _ in range(16):
             for kind, h in (("p", lam * eps), ("u", eps / 2), ("p", (1 - 2 * lam) * eps), ("u", eps / 2),
                             ("p", lam * eps)):
-                if kind == "p":
+                if kind == "p" and f32force:
+                    L32.upload_links(L64.download_links())
+                    L32.stout_force(COEFF)
+                    L64.upload_momenta(L64.download_momenta() + h * L32.download_force())
+                elif kind == "p":
                     L64.force_kick(COEFF, h)
                     if rm:
                         L64.round_to_float(momenta=True)
@@ -349,7 +366,7 @@ def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232):
 
     variants = {"round_momenta": (False, True, True), "round_links": (True, False, True),
                 "round_links_noproj": (True, False, False), "round_both": (True, True, True)}
-    dev = {k: [] for k in ["fp32_bulk", "fp64_fused"] + list(variants)}
+    dev = {k: [] for k in ["fp32_bulk", "fp64_fused"] + list(variants) + ["fp32_forces"]}
     ref = []
     t0 = time.perf_counter()
     for d in range(ndraws):
@@ -369,7 +386,11 @@ def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232):
         L32.upload_links(U0)
         L32.upload_momenta(P0)
         dev["fp32_bulk"].append(L32.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True)[2] - dh64)
-        log(f"[precision] draw {d}: dH64 {dh64:.5g} " + " ".join(f"{k} {v[-1]:+.2e}" for k, v in dev.items()))
+        if d < nf32:
+            L64.upload_links(U0)
+            L64.upload_momenta(P0)
+            dev["fp32_forces"].append(stepped(False, False, f32force=True) - dh64)
+        log(f"[precision] draw {d}: dH64 {dh64:.5g} " + " ".join(f"{k} {v[-1]:+.2e}" for k, v in dev.items() if v))
     L64.close()
     L32.close()
 
@@ -384,7 +405,7 @@ def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232):
             "tau": 1.0, "draws": ndraws, "thermalization_dH": therm,
             "metropolis_acceptance": acc / nmetro if nmetro else None, "plaquette_start": plaq0,
             "dH_fp64": {"rms": float(np.sqrt(np.mean(np.square(ref)))), "values": ref},
-            "deviation_from_fp64": {k: stats(v) for k, v in dev.items()},
+            "deviation_from_fp64": {k: stats(v) for k, v in dev.items() if v},
             "deviations": dev, "wall_s": time.perf_counter() - t0}
 
 

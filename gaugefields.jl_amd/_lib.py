@@ -9,7 +9,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgfhip.so")
 
-GF_F64, GF_F32 = 0, 1
+GF_F64, GF_F32, GF_F32_FORCE = 0, 1, 2
 GF_OK, GF_EINVAL, GF_EDEVICE, GF_ENOMEM, GF_ESTATE = 0, -1, -2, -3, -4
 GF_QPQ, GF_PQP, GF_OMELYAN2MN, GF_FUSED = 0, 1, 2, 0x100
 GF_STOUT_STORE, GF_STOUT_RECOMPUTE, GF_STOUT_AUTO = 0, 1, 2

@@ -117,7 +117,8 @@ class HIPGaugeConfiguration:
 
     @property
     def eltype(self):
-        return np.complex128 if self.lattice.precision == "f64" else np.complex64
+        # the links as held: FP64 also in the FP32-force form, whose forces alone are FP32
+        return np.complex64 if self.lattice.precision == "f32" else np.complex128
 
     def to_host(self):
         """links as (4, V, 9) complex128, reference layout"""
@@ -164,12 +165,16 @@ def _check_same(U, P):
 
 
 def gauge_configuration(lattice, *, backend=None, colors=3, halo=1, start="cold", seed=None,
-                        eltype=np.complex128, boundary="periodic", verbose=0, device=0):
+                        eltype=np.complex128, boundary="periodic", verbose=0, device=0, force_eltype=None):
     """Create a configuration on the GPU (gauge_configuration, API.jl:178-251).
 
     start="cold": unit links.  start="hot": the hot fill of the legacy backend (entries
     (u - 1/2) + i (u' - 1/2), rows Gram-Schmidt normalized to SU(3)) generated on the device from
-    the Philox4x32-10 stream of ``seed``; ``seed=None`` draws a fresh seed."""
+    the Philox4x32-10 stream of ``seed``; ``seed=None`` draws a fresh seed.
+
+    force_eltype (HIP backend only, no counterpart in the reference): np.complex64 with
+    eltype=np.complex128 keeps the links and momenta in FP64 and evaluates every force in FP32
+    (the "f32force" form)."""
     if backend is not None and not isinstance(backend, HIPBackend):
         raise ArgumentError("only HIPBackend() is available here")
     dims = tuple(int(d) for d in lattice)
@@ -187,7 +192,12 @@ def gauge_configuration(lattice, *, backend=None, colors=3, halo=1, start="cold"
         raise ArgumentError("the HIP backend supports periodic boundaries only")
     if eltype not in (np.complex128, np.complex64):
         raise ArgumentError("eltype must be ComplexF64 (np.complex128) or ComplexF32 (np.complex64)")
-    L = Lattice(dims, precision="f64" if eltype == np.complex128 else "f32", device=device)
+    if force_eltype not in (None, eltype, np.complex64):
+        raise ArgumentError("force_eltype must be the links' eltype or ComplexF32 (np.complex64)")
+    precision = "f64" if eltype == np.complex128 else "f32"
+    if precision == "f64" and force_eltype == np.complex64:
+        precision = "f32force"
+    L = Lattice(dims, precision=precision, device=device)
     U = HIPGaugeConfiguration(L)
     if start == "hot":
         if seed is None:

@@ -23,7 +23,9 @@ struct gf_ctx {
     int dims[4] = {0, 0, 0, 0};
     Geo geo{};
     int V = 0;
-    int prec = GF_F64;
+    int prec = GF_F64;       // arithmetic of the kernels: GF_F64 or GF_F32
+    int pub_prec = GF_F64;   // as created (GF_F32_FORCE: GF_F32 kernels over an FP64 state)
+    bool s64 = false;        // FP64 MD state (U64, P64) behind FP32 forces
     int device = 0;
     size_t esz = 8;  // bytes per real
     hipStream_t stream = nullptr;
@@ -46,6 +48,11 @@ struct gf_ctx {
     void* G2 = nullptr;  // second Lam buffer (stored-C mode: levels chained across kernels)
     // FP32 bulk: FP64 copies of the links (and smeared levels, ping-pong) for the Hamiltonian
     void* W64[2] = {nullptr, nullptr};
+    // FP32 forces over an FP64 state: the links and momenta as held and updated (U holds the links
+    // rounded to float, which the force kernels read), and the Metropolis backup
+    void* U64 = nullptr;
+    void* P64 = nullptr;
+    void* U64bak = nullptr;
     // scratch for layout conversion and reductions
     double* staging = nullptr;  // 72 V doubles
     double* partial = nullptr;
@@ -222,6 +229,10 @@ void swap_links(gf_ctx* c) { std::swap(c->U, c->Ualt); }
 // field-strength observables of the current links: out = {clover energy sum, Q plaquette,
 // Q clover, Q rectangle}; dens_host (V doubles) receives the density of dens_method if non-null
 int fstrength(gf_ctx* c, int sets, double out[4], double* dens_host, int dens_method) {
+    if (c->s64)
+        return fail(c, GF_ESTATE,
+                    "field-strength measurements are not available in the FP32-force form (FP64 links and momenta, "
+                    "forces in FP32); use a GF_F64 or GF_F32 context");
     double* dens = nullptr;
     if (dens_host) {
         TRY(need_staging(c));
@@ -339,7 +350,34 @@ int stout_force_impl(gf_ctx* c, double coeff, int mode, double eps, double eps_u
     return stout_backward_all(c, mode, target, eps, eps_u, true, coeff);
 }
 
+// ---- FP32 forces over an FP64 MD state ---------------------------------------------------
+size_t link64_bytes(const gf_ctx* c) { return (size_t)48 * c->V * sizeof(double); }
+int state64_step(gf_ctx* c, int mode, double eps_p, double eps_u) {
+    HIPCHK(c, launch_state64_step(mode, c->geo, (const float*)c->F, (double*)c->P64, (double*)c->U64, (float*)c->U,
+                                  eps_p, eps_u, c->stream));
+    return GF_OK;
+}
+// the force of the links rounded to float, FP32 throughout, into F
+int force_f32(gf_ctx* c, double coeff) {
+    if (c->nlev > 0) return stout_force_impl(c, coeff, FORCE_STORE, 0.0);
+    HIPCHK(c, launch_force<float>(FORCE_STORE, c->geo, (const float*)c->U, (float*)c->F, nullptr, coeff, 0.0, 0.0,
+                                  c->stream));
+    return GF_OK;
+}
+int not_state64(gf_ctx* c, const char* what) {
+    return fail(c, GF_ESTATE,
+                "%s is not available in the FP32-force form (FP64 links and momenta, forces in FP32); use a GF_F64 or "
+                "GF_F32 context",
+                what);
+}
+#define NOT_S64(c, what) \
+    if ((c)->s64) return not_state64(c, what)
+
 int kick(gf_ctx* c, double coeff, double eps) {
+    if (c->s64) {
+        TRY(force_f32(c, coeff));
+        return state64_step(c, 0, eps, 0.0);
+    }
     if (c->nlev > 0) return stout_force_impl(c, coeff, FORCE_KICK, eps);
     return dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
@@ -349,6 +387,7 @@ int kick(gf_ctx* c, double coeff, double eps) {
 }
 
 int expupd(gf_ctx* c, double eps) {
+    if (c->s64) return state64_step(c, 1, 0.0, eps);
     return dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
         HIPCHK(c, launch_exp_update<T>(c->geo, (const T*)c->U, (T*)c->U, (const T*)c->P, eps, c->stream));
@@ -358,6 +397,10 @@ int expupd(gf_ctx* c, double eps) {
 
 // kick then exp update; one fused kernel for the plain action
 int kick_exp(gf_ctx* c, double coeff, double eps_p, double eps_u) {
+    if (c->s64) {
+        TRY(force_f32(c, coeff));
+        return state64_step(c, 2, eps_p, eps_u);
+    }
     if (c->nlev > 0) {  // the thin-link backward stencil kicks and updates (into Ualt)
         TRY(stout_force_impl(c, coeff, FORCE_KICK_EXP, eps_p, eps_u));
         swap_links(c);
@@ -377,31 +420,48 @@ int kick_exp(gf_ctx* c, double coeff, double eps_p, double eps_u) {
 // widened to FP64 and smeared in FP64 (no C kept): the FP32 smearing chain's own rounding would
 // add configuration-dependent noise of ~2e-3 to the smeared action at 32^4, more than the
 // deviation the FP32 trajectory itself causes; the forces stay FP32.
+// With an FP64 state the held FP64 links are smeared directly.
 int action_plaq_fp64(gf_ctx* c, double* plaq) {
     if (c->prec == GF_F64) {
         if (c->nlev > 0) TRY(stout_forward_all(c));
         return plaq_of(c, (void*)top_links(c), plaq);
     }
-    const size_t b64 = (size_t)48 * c->V * sizeof(double);
-    TRY(alloc(c, &c->W64[0], b64));
-    if (c->nlev > 0) TRY(alloc(c, &c->W64[1], b64));
-    HIPCHK(c, launch_links_widen((const float*)c->U, (double*)c->W64[0], c->V, c->stream));
-    int cur = 0;
-    for (int l = 0; l < c->nlev; l++, cur ^= 1)
-        HIPCHK(c, launch_stout_forward<double>(c->geo, (const double*)c->W64[cur], (double*)c->W64[cur ^ 1], nullptr,
-                                               c->rho, c->stream));
-    HIPCHK(c, launch_plaquette<double>(c->geo, (const double*)c->W64[cur], c->partial, c->dres, c->stream));
+    const size_t b64 = link64_bytes(c);
+    if (!c->s64 || c->nlev > 0) TRY(alloc(c, &c->W64[0], b64));
+    if (c->nlev > 1 || (!c->s64 && c->nlev > 0)) TRY(alloc(c, &c->W64[1], b64));
+    if (!c->s64) HIPCHK(c, launch_links_widen((const float*)c->U, (double*)c->W64[0], c->V, c->stream));
+    // level l reads src and writes W64[l & 1] (the FP64 state is never overwritten)
+    const double* src = c->s64 ? (const double*)c->U64 : (const double*)c->W64[0];
+    int cur = c->s64 ? 0 : 1;
+    for (int l = 0; l < c->nlev; l++, cur ^= 1) {
+        HIPCHK(c, launch_stout_forward<double>(c->geo, src, (double*)c->W64[cur], nullptr, c->rho, c->stream));
+        src = (const double*)c->W64[cur];
+    }
+    HIPCHK(c, launch_plaquette<double>(c->geo, src, c->partial, c->dres, c->stream));
     return read_scalar(c, plaq);
+}
+// plaquette sum of the configuration's links (the FP64 state where there is one)
+int plaq_state(gf_ctx* c, double* sum) {
+    if (!c->s64) return plaq_of(c, c->U, sum);
+    HIPCHK(c, launch_plaquette<double>(c->geo, (const double*)c->U64, c->partial, c->dres, c->stream));
+    return read_scalar(c, sum);
+}
+int momenta_dot(gf_ctx* c, double* pp) {
+    if (c->s64) {
+        HIPCHK(c, launch_momenta_dot<double>(c->geo, (const double*)c->P64, c->partial, c->dres, c->stream));
+    } else {
+        TRY(dispatch(c, [&](auto* tag) -> int {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            HIPCHK(c, launch_momenta_dot<T>(c->geo, (const T*)c->P, c->partial, c->dres, c->stream));
+            return GF_OK;
+        }));
+    }
+    return read_scalar(c, pp);
 }
 
 int hamiltonian(gf_ctx* c, double coeff, double* H) {
     double pp = 0.0, plaq = 0.0;
-    TRY(dispatch(c, [&](auto* tag) -> int {
-        using T = std::remove_pointer_t<decltype(tag)>;
-        HIPCHK(c, launch_momenta_dot<T>(c->geo, (const T*)c->P, c->partial, c->dres, c->stream));
-        return GF_OK;
-    }));
-    TRY(read_scalar(c, &pp));
+    TRY(momenta_dot(c, &pp));
     TRY(action_plaq_fp64(c, &plaq));
     *H = -2.0 * coeff * plaq / 3.0 + 0.5 * pp;
     return GF_OK;
@@ -443,7 +503,7 @@ const char* gf_last_error(const gf_ctx* ctx) { return ctx ? ctx->err.c_str() : g
 int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
     if (!dims || !out) return GF_EINVAL;
     *out = nullptr;
-    if (precision != GF_F64 && precision != GF_F32) return GF_EINVAL;
+    if (precision != GF_F64 && precision != GF_F32 && precision != GF_F32_FORCE) return GF_EINVAL;
     long long V = 1;
     for (int m = 0; m < 4; m++) {
         if (dims[m] <= 0) return GF_EINVAL;
@@ -460,8 +520,10 @@ int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
         std::vector<char> seen((size_t)(V / 64) + 1);
         make_geometry(dims, V, c->geo, seen.data());
     }
-    c->prec = precision;
-    c->esz = precision == GF_F64 ? 8 : 4;
+    c->pub_prec = precision;
+    c->s64 = precision == GF_F32_FORCE;
+    c->prec = c->s64 ? GF_F32 : precision;
+    c->esz = c->prec == GF_F64 ? 8 : 4;
     c->device = device;
     auto bail = [&](int code) {
         gf_destroy(c);
@@ -476,6 +538,8 @@ int gf_create(const int* dims, int precision, int device, gf_ctx** out) {
         alloc(c, (void**)&c->dres, 4 * sizeof(double)))
         return bail(GF_ENOMEM);
     if (hipHostMalloc((void**)&c->hres, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) return bail(GF_ENOMEM);
+    if (c->s64 && (alloc(c, &c->U64, link64_bytes(c)) || alloc(c, &c->P64, (size_t)32 * c->V * sizeof(double))))
+        return bail(GF_ENOMEM);
     int r = gf_set_cold(c);
     if (r == GF_OK) r = gf_clear_momenta(c);
     if (r == GF_OK) r = gf_sync(c);
@@ -488,7 +552,8 @@ int gf_destroy(gf_ctx* c) {
     if (!c) return GF_EINVAL;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void** bufs[] = {&c->U, &c->Ualt, &c->Ubak, &c->P, &c->F, &c->A, &c->D, &c->D2, &c->G, &c->G2, &c->W64[0], &c->W64[1]};
+    void** bufs[] = {&c->U,   &c->Ualt, &c->Ubak, &c->P,      &c->F,      &c->A,   &c->D,   &c->D2,
+                     &c->G,   &c->G2,   &c->W64[0], &c->W64[1], &c->U64, &c->P64, &c->U64bak};
     for (void** b : bufs)
         if (*b) (void)hipFree(*b);
     for (int l = 0; l < GF_MAX_LEV; l++) {
@@ -516,13 +581,14 @@ int gf_info(const gf_ctx* c, int* dims, int* precision, size_t* device_bytes) {
     if (!c) return GF_EINVAL;
     if (dims)
         for (int m = 0; m < 4; m++) dims[m] = c->dims[m];
-    if (precision) *precision = c->prec;
+    if (precision) *precision = c->pub_prec;
     if (device_bytes) *device_bytes = c->bytes;
     return GF_OK;
 }
 
 int gf_set_cold(gf_ctx* c) {
     CTXCHK(c);
+    if (c->s64) HIPCHK(c, launch_set_identity<double>((double*)c->U64, c->V, c->stream));
     return dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
         HIPCHK(c, launch_set_identity<T>((T*)c->U, c->V, c->stream));
@@ -589,11 +655,16 @@ int gf_upload_links(gf_ctx* c, const double* ref) {
     }
     TRY(need_staging(c));
     HIPCHK(c, hipMemcpyAsync(c->staging, ref, (size_t)72 * c->V * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    TRY(dispatch(c, [&](auto* tag) -> int {
-        using T = std::remove_pointer_t<decltype(tag)>;
-        HIPCHK(c, launch_links_from_ref<T>(c->staging, (T*)c->U, c->V, false, c->stream));
-        return GF_OK;
-    }));
+    if (c->s64) {
+        HIPCHK(c, launch_links_from_ref<double>(c->staging, (double*)c->U64, c->V, false, c->stream));
+        TRY(state64_step(c, 3, 0.0, 0.0));
+    } else {
+        TRY(dispatch(c, [&](auto* tag) -> int {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            HIPCHK(c, launch_links_from_ref<T>(c->staging, (T*)c->U, c->V, false, c->stream));
+            return GF_OK;
+        }));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GF_OK;
 }
@@ -626,12 +697,25 @@ static int upload_linkfield(gf_ctx* c, void* field, const double* ref, bool gene
 int gf_download_links(gf_ctx* c, double* ref) {
     CTXCHK(c);
     if (!ref) return fail(c, GF_EINVAL, "null links");
+    if (c->s64) {
+        TRY(need_staging(c));
+        HIPCHK(c, launch_links_to_ref<double>((const double*)c->U64, c->staging, c->V, false, c->stream));
+        HIPCHK(c, hipMemcpyAsync(ref, c->staging, (size_t)72 * c->V * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return GF_OK;
+    }
     return download_linkfield(c, c->U, ref);
 }
 
-static int upload_mom(gf_ctx* c, void* field, const double* mom) {
+// f64: the field is an FP64 momentum field whatever the context's arithmetic
+static int upload_mom(gf_ctx* c, void* field, const double* mom, bool f64 = false) {
     TRY(need_staging(c));
     HIPCHK(c, hipMemcpyAsync(c->staging, mom, (size_t)32 * c->V * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (f64) {
+        HIPCHK(c, launch_momenta_from_ref<double>(c->staging, (double*)field, c->V, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return GF_OK;
+    }
     TRY(dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
         HIPCHK(c, launch_momenta_from_ref<T>(c->staging, (T*)field, c->V, c->stream));
@@ -640,13 +724,17 @@ static int upload_mom(gf_ctx* c, void* field, const double* mom) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GF_OK;
 }
-static int download_mom(gf_ctx* c, const void* field, double* mom) {
+static int download_mom(gf_ctx* c, const void* field, double* mom, bool f64 = false) {
     TRY(need_staging(c));
-    TRY(dispatch(c, [&](auto* tag) -> int {
-        using T = std::remove_pointer_t<decltype(tag)>;
-        HIPCHK(c, launch_momenta_to_ref<T>((const T*)field, c->staging, c->V, c->stream));
-        return GF_OK;
-    }));
+    if (f64) {
+        HIPCHK(c, launch_momenta_to_ref<double>((const double*)field, c->staging, c->V, c->stream));
+    } else {
+        TRY(dispatch(c, [&](auto* tag) -> int {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            HIPCHK(c, launch_momenta_to_ref<T>((const T*)field, c->staging, c->V, c->stream));
+            return GF_OK;
+        }));
+    }
     HIPCHK(c, hipMemcpyAsync(mom, c->staging, (size_t)32 * c->V * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return GF_OK;
@@ -655,12 +743,12 @@ static int download_mom(gf_ctx* c, const void* field, double* mom) {
 int gf_upload_momenta(gf_ctx* c, const double* mom) {
     CTXCHK(c);
     if (!mom) return fail(c, GF_EINVAL, "null momenta");
-    return upload_mom(c, c->P, mom);
+    return c->s64 ? upload_mom(c, c->P64, mom, true) : upload_mom(c, c->P, mom);
 }
 int gf_download_momenta(gf_ctx* c, double* mom) {
     CTXCHK(c);
     if (!mom) return fail(c, GF_EINVAL, "null momenta");
-    return download_mom(c, c->P, mom);
+    return c->s64 ? download_mom(c, c->P64, mom, true) : download_mom(c, c->P, mom);
 }
 int gf_download_force(gf_ctx* c, double* force) {
     CTXCHK(c);
@@ -669,6 +757,7 @@ int gf_download_force(gf_ctx* c, double* force) {
 }
 int gf_clear_momenta(gf_ctx* c) {
     CTXCHK(c);
+    if (c->s64) HIPCHK(c, launch_fill<double>((double*)c->P64, (size_t)32 * c->V, 0.0, c->stream));
     return dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
         HIPCHK(c, launch_fill<T>((T*)c->P, (size_t)32 * c->V, T(0), c->stream));
@@ -678,12 +767,22 @@ int gf_clear_momenta(gf_ctx* c) {
 
 int gf_save_links(gf_ctx* c) {
     CTXCHK(c);
+    if (c->s64) {
+        TRY(alloc(c, &c->U64bak, link64_bytes(c)));
+        HIPCHK(c, hipMemcpyAsync(c->U64bak, c->U64, link64_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+        return GF_OK;
+    }
     TRY(alloc(c, &c->Ubak, link_bytes(c)));
     HIPCHK(c, hipMemcpyAsync(c->Ubak, c->U, link_bytes(c), hipMemcpyDeviceToDevice, c->stream));
     return GF_OK;
 }
 int gf_restore_links(gf_ctx* c) {
     CTXCHK(c);
+    if (c->s64) {
+        if (!c->U64bak) return fail(c, GF_ESTATE, "gf_restore_links before gf_save_links");
+        HIPCHK(c, hipMemcpyAsync(c->U64, c->U64bak, link64_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+        return state64_step(c, 3, 0.0, 0.0);
+    }
     if (!c->Ubak) return fail(c, GF_ESTATE, "gf_restore_links before gf_save_links");
     HIPCHK(c, hipMemcpyAsync(c->U, c->Ubak, link_bytes(c), hipMemcpyDeviceToDevice, c->stream));
     return GF_OK;
@@ -692,18 +791,19 @@ int gf_restore_links(gf_ctx* c) {
 int gf_plaquette(gf_ctx* c, double* sum) {
     CTXCHK(c);
     if (!sum) return GF_EINVAL;
-    return plaq_of(c, c->U, sum);
+    return plaq_state(c, sum);
 }
 int gf_action(gf_ctx* c, double coeff, double* value) {
     CTXCHK(c);
     if (!value) return GF_EINVAL;
     double s;
-    TRY(plaq_of(c, c->U, &s));
+    TRY(plaq_state(c, &s));
     *value = 2.0 * coeff * s;
     return GF_OK;
 }
 int gf_action_untraced(gf_ctx* c, double coeff, double* ref) {
     CTXCHK(c);
+    NOT_S64(c, "evaluate_GaugeAction_untraced");
     if (!ref) return fail(c, GF_EINVAL, "the output array must not be null");
     TRY(need_staging(c));
     TRY(dispatch(c, [&](auto* tag) -> int {
@@ -718,12 +818,7 @@ int gf_action_untraced(gf_ctx* c, double coeff, double* ref) {
 int gf_momenta_dot(gf_ctx* c, double* value) {
     CTXCHK(c);
     if (!value) return GF_EINVAL;
-    TRY(dispatch(c, [&](auto* tag) -> int {
-        using T = std::remove_pointer_t<decltype(tag)>;
-        HIPCHK(c, launch_momenta_dot<T>(c->geo, (const T*)c->P, c->partial, c->dres, c->stream));
-        return GF_OK;
-    }));
-    return read_scalar(c, value);
+    return momenta_dot(c, value);
 }
 int gf_hamiltonian(gf_ctx* c, double coeff, double* H) {
     CTXCHK(c);
@@ -733,6 +828,7 @@ int gf_hamiltonian(gf_ctx* c, double coeff, double* H) {
 
 int gf_dsdu(gf_ctx* c, double coeff, double* ref) {
     CTXCHK(c);
+    NOT_S64(c, "calc_dSdUmu");
     TRY(alloc(c, &c->D, mat_bytes(c)));
     TRY(dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
@@ -827,7 +923,7 @@ int gf_energy_density(gf_ctx* c, double* E) {
     CTXCHK(c);
     if (!E) return GF_EINVAL;
     double s;
-    TRY(plaq_of(c, c->U, &s));
+    TRY(plaq_state(c, &s));
     *E = 2.0 * (18.0 - s / (double)c->V);
     return GF_OK;
 }
@@ -835,6 +931,10 @@ int gf_energy_density(gf_ctx* c, double* E) {
 int gf_gaussian_momenta(gf_ctx* c, double sigma, unsigned long long seed, unsigned long long sweep) {
     CTXCHK(c);
     if (!(sigma >= 0.0) || !std::isfinite(sigma)) return fail(c, GF_EINVAL, "sigma must be finite and >= 0; got %g", sigma);
+    if (c->s64) {
+        HIPCHK(c, launch_gaussian_momenta<double>(c->geo, (double*)c->P64, sigma, seed, sweep, c->stream));
+        return GF_OK;
+    }
     return dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
         HIPCHK(c, launch_gaussian_momenta<T>(c->geo, (T*)c->P, sigma, seed, sweep, c->stream));
@@ -844,6 +944,10 @@ int gf_gaussian_momenta(gf_ctx* c, double sigma, unsigned long long seed, unsign
 
 int gf_hot_start(gf_ctx* c, unsigned long long seed) {
     CTXCHK(c);
+    if (c->s64) {
+        HIPCHK(c, launch_hot_links<double>(c->geo, (double*)c->U64, seed, c->stream));
+        return state64_step(c, 3, 0.0, 0.0);
+    }
     return dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
         HIPCHK(c, launch_hot_links<T>(c->geo, (T*)c->U, seed, c->stream));
@@ -853,6 +957,7 @@ int gf_hot_start(gf_ctx* c, unsigned long long seed) {
 
 static int heatbath_sweep(gf_ctx* c, int mode, double beta, unsigned long long seed, unsigned long long sweep,
                           int itmax, long long* failures) {
+    NOT_S64(c, mode == 0 ? "heatbath!" : "overrelaxation!");
     for (int m = 0; m < 4; m++)
         if (c->geo.n[m] % 2)
             return fail(c, GF_EINVAL, "the checkerboard heatbath needs even lattice extents; got %d in direction %d",
@@ -915,6 +1020,7 @@ int gf_energy_density_clover(gf_ctx* c, double* E) {
 
 int gf_flow_measure(gf_ctx* c, double eps, int nsteps, int every, double* out) {
     CTXCHK(c);
+    NOT_S64(c, "flow!");
     if (every <= 0) return fail(c, GF_EINVAL, "every must be positive; got %d", every);
     if (!out) return fail(c, GF_EINVAL, "out is required");
     int k = 0;
@@ -941,6 +1047,7 @@ int gf_flow_measure(gf_ctx* c, double eps, int nsteps, int every, double* out) {
 
 int gf_flow(gf_ctx* c, double eps, int nsteps, int every, double* E_out) {
     CTXCHK(c);
+    NOT_S64(c, "flow!");
     if (nsteps <= 0) return fail(c, GF_EINVAL, "steps must be positive; got %d", nsteps);
     if (!(eps > 0.0) || !std::isfinite(eps)) return fail(c, GF_EINVAL, "step_size must be positive; got %g", eps);
     if (every > 0 && !E_out) return fail(c, GF_EINVAL, "E_out is required when every > 0");
@@ -1000,6 +1107,7 @@ int gf_stout_mode(const gf_ctx* c, int* mode) {
 
 int gf_stout_smear(gf_ctx* c, double* ref) {
     CTXCHK(c);
+    NOT_S64(c, "calc_smearedU");
     if (c->nlev == 0) return fail(c, GF_ESTATE, "no stout levels configured");
     TRY(stout_forward_all(c));
     if (ref) return download_linkfield(c, c->lev[c->nlev - 1], ref);
@@ -1008,6 +1116,7 @@ int gf_stout_smear(gf_ctx* c, double* ref) {
 
 int gf_stout_backward(gf_ctx* c, const double* dout_ref, double* din_ref) {
     CTXCHK(c);
+    NOT_S64(c, "back_prop");
     if (c->nlev == 0) return fail(c, GF_ESTATE, "no stout levels configured");
     if (!dout_ref || !din_ref) return fail(c, GF_EINVAL, "null derivative buffer");
     TRY(upload_linkfield(c, c->D, dout_ref, true));

@@ -48,6 +48,9 @@ hipError_t launch_exp_update(const Geo& g, const T* Uin, T* Uout, const T* P, do
 // reference-layout (column-major 3x3 per link) <-> device field; general = false: SU(3) link
 // field (rows 0 and 1 kept, row 2 rebuilt on the way out), true: general matrix field
 hipError_t launch_links_widen(const float* Uf, double* Ud, int V, hipStream_t st);
+// FP32 forces over an FP64 MD state (k_state64_step): mode 0 kick, 1 exp update, 2 both, 3 rounding
+hipError_t launch_state64_step(int mode, const Geo& g, const float* F, double* P64, double* U64, float* U32,
+                               double eps_p, double eps_u, hipStream_t st);
 template <typename T>
 hipError_t launch_links_from_ref(const double* ref, T* soa, int V, bool general, hipStream_t st);
 template <typename T>

@@ -266,6 +266,53 @@ __global__ __launch_bounds__(kBlock) void k_links_widen(const float* __restrict_
     }
 }
 
+// FP32 forces over an FP64 MD state: the links U64 and momenta P64 are held and updated in double;
+// the force chain reads U32, the links rounded to float, and leaves its force in F (float).
+// MODE 0: P64 += eps_p F.  1: U64 <- exp(eps_u P64) U64 (in place: only the thread's own link is
+// read) and U32 <- round(U64).  2: both (kick, then update).  3: U32 <- round(U64).
+// One thread per site walking its four links: a streaming kernel.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_state64_step(Geo g, const float* __restrict__ F, double* __restrict__ P64,
+                                                         double* __restrict__ U64, float* __restrict__ U32,
+                                                         double eps_p, double eps_u) {
+    const int s = site_index();
+    if (s >= g.V) return;
+    const unsigned V = g.V;
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) {
+        if (MODE == 3) {
+            stu<float, GF_ST_NT>(U32, V, mu, s, round_link<float>(ldu<double, GF_LD_NT>(U64, V, mu, s)));
+            continue;
+        }
+        double p[8];
+        ldp<double, GF_LD_NT>(P64, V, mu, s, p);
+        if (MODE != 1) {
+            float f[8];
+            ldp<float, GF_LD_NT>(F, V, mu, s, f);
+#pragma unroll
+            for (int a = 0; a < 8; a++) p[a] = fma(eps_p, (double)f[a], p[a]);
+            stp<double, GF_ST_NT>(P64, V, mu, s, p);
+        }
+        if (MODE != 0) {
+            const M3<double> un = exp_coeffs_times(p, eps_u, ldu<double, GF_LD_NT>(U64, V, mu, s));
+            stu<double, GF_ST_NT>(U64, V, mu, s, un);
+            stu<float, GF_ST_NT>(U32, V, mu, s, round_link<float>(un));
+        }
+    }
+}
+
+hipError_t launch_state64_step(int mode, const Geo& g, const float* F, double* P64, double* U64, float* U32,
+                               double eps_p, double eps_u, hipStream_t st) {
+    const int nb = grid_for(g.V);
+    switch (mode) {
+        case 0: k_state64_step<0><<<nb, kBlock, 0, st>>>(g, F, P64, U64, U32, eps_p, eps_u); break;
+        case 1: k_state64_step<1><<<nb, kBlock, 0, st>>>(g, F, P64, U64, U32, eps_p, eps_u); break;
+        case 2: k_state64_step<2><<<nb, kBlock, 0, st>>>(g, F, P64, U64, U32, eps_p, eps_u); break;
+        default: k_state64_step<3><<<nb, kBlock, 0, st>>>(g, F, P64, U64, U32, eps_p, eps_u); break;
+    }
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // layout conversion between the reference's (3,3,X,Y,Z,T) complex column-major arrays
 // (FP64, per direction) and the SoA planes

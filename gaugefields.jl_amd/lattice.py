@@ -13,7 +13,9 @@ from .generators import volume
 _D = ctypes.POINTER(ctypes.c_double)
 
 INTEGRATORS = {"QPQ": _lib.GF_QPQ, "PQP": _lib.GF_PQP, "2MN": _lib.GF_OMELYAN2MN}
-PRECISIONS = {"f64": _lib.GF_F64, "f32": _lib.GF_F32}
+# "f32force": links and momenta held and updated in FP64, every force evaluated in FP32 from the
+# links rounded to float (GF_F32_FORCE)
+PRECISIONS = {"f64": _lib.GF_F64, "f32": _lib.GF_F32, "f32force": _lib.GF_F32_FORCE}
 #: Omelyan 2MN parameter (Omelyan, Mryglod, Folk, Comput. Phys. Commun. 151 (2003) 272)
 OMELYAN_LAMBDA = 0.1931833275037836
 
@@ -29,7 +31,7 @@ class Lattice:
         self.dims = tuple(int(d) for d in dims)
         self.V = volume(self.dims)
         if precision not in PRECISIONS:
-            raise ValueError(f"precision must be 'f64' or 'f32'; got {precision!r}")
+            raise ValueError(f"precision must be 'f64', 'f32' or 'f32force'; got {precision!r}")
         self.precision = precision
         self._L = _lib.load()
         ctx = ctypes.c_void_p()

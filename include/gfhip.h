@@ -21,7 +21,12 @@
  *
  * Precision: GF_F64 stores and computes links/momenta in double.  GF_F32 ("FP32 bulk")
  * stores and computes them in float while every global sum (plaquette, action, p*p,
- * Hamiltonian) and every Cayley-Hamilton coefficient is evaluated in double.
+ * Hamiltonian) and every Cayley-Hamilton coefficient is evaluated in double.  GF_F32_FORCE
+ * ("FP32 forces") holds and updates the links and momenta in double (the MD state, the link
+ * updates and the Hamiltonian as in GF_F64) and evaluates every force -- staples, stout chain,
+ * TA projection -- in float from the links rounded to float (as GF_F32 does); the flow, the
+ * heatbath, the field-strength measurements and the explicit stout smearing/backward calls are
+ * not available in this form (GF_ESTATE).
  */
 #ifndef GFHIP_H
 #define GFHIP_H
@@ -34,7 +39,7 @@ extern "C" {
 
 typedef struct gf_ctx gf_ctx;
 
-enum { GF_F64 = 0, GF_F32 = 1 };
+enum { GF_F64 = 0, GF_F32 = 1, GF_F32_FORCE = 2 };
 enum { GF_OK = 0, GF_EINVAL = -1, GF_EDEVICE = -2, GF_ENOMEM = -3, GF_ESTATE = -4 };
 enum { GF_QPQ = 0, GF_PQP = 1, GF_OMELYAN2MN = 2, GF_FUSED = 0x100 };
 enum { GF_STOUT_STORE = 0, GF_STOUT_RECOMPUTE = 1, GF_STOUT_AUTO = 2 };
