@@ -317,7 +317,7 @@ int stout_backward_all(gf_ctx* c, int fmode = -1, void* FP = nullptr, double eps
             for (int l = top; l >= 0; l--) {
                 HIPCHK(c, launch_stout_backward<T>(c->geo, (const T*)level_in(c, l), nullptr, (const T*)c->D, (T*)c->D2,
                                                    (T*)c->G, c->rho, c->stream, l == 0 ? fmode : -1, (T*)FP, eps,
-                                                   (T*)c->Ualt, eps_u));
+                                                   (T*)c->Ualt, eps_u, (double*)c->U64));
                 std::swap(c->D, c->D2);
             }
             return GF_OK;
@@ -334,19 +334,22 @@ int stout_backward_all(gf_ctx* c, int fmode = -1, void* FP = nullptr, double eps
                                                  (const T*)level_in(c, l - 1), (const T*)c->Cst[l - 1],
                                                  lam[(l - 1) & 1], c->stream));
         HIPCHK(c, launch_stout_back_stencil<T>(c->geo, (const T*)level_in(c, 0), lam[0], (T*)c->D2, c->rho, c->stream,
-                                               fmode, (T*)FP, eps, (T*)c->Ualt, eps_u));
+                                               fmode, (T*)FP, eps, (T*)c->Ualt, eps_u, (double*)c->U64));
         std::swap(c->D, c->D2);
         return GF_OK;
     });
 }
 
 // mode FORCE_STORE -> F, FORCE_KICK -> P += eps F, FORCE_KICK_EXP -> that and Ualt = exp(eps_u P) U
-// (the caller swaps the link buffers)
+// (the caller swaps the link buffers); FORCE_KICK_S64 / FORCE_KICK_EXP_S64 the same on the FP64
+// momenta and links of an FP64 state (Ualt: the updated links rounded to float)
 // into_force: FORCE_KICK into the force field F instead of the momenta (F += eps force)
 int stout_force_impl(gf_ctx* c, double coeff, int mode, double eps, double eps_u = 0.0, bool into_force = false) {
     TRY(stout_forward_all(c));
     // the thin-link level's stencil kernel forms the force from dS/dU_bare
-    void* target = (mode == FORCE_STORE || into_force) ? (void*)c->F : (void*)c->P;
+    void* target = (mode == FORCE_STORE || into_force) ? (void*)c->F
+                   : (mode == FORCE_KICK_S64 || mode == FORCE_KICK_EXP_S64) ? c->P64
+                                                                          : (void*)c->P;
     return stout_backward_all(c, mode, target, eps, eps_u, true, coeff);
 }
 
@@ -375,6 +378,7 @@ int not_state64(gf_ctx* c, const char* what) {
 
 int kick(gf_ctx* c, double coeff, double eps) {
     if (c->s64) {
+        if (c->nlev > 0) return stout_force_impl(c, coeff, FORCE_KICK_S64, eps);
         TRY(force_f32(c, coeff));
         return state64_step(c, 0, eps, 0.0);
     }
@@ -398,6 +402,11 @@ int expupd(gf_ctx* c, double eps) {
 // kick then exp update; one fused kernel for the plain action
 int kick_exp(gf_ctx* c, double coeff, double eps_p, double eps_u) {
     if (c->s64) {
+        if (c->nlev > 0) {  // the thin-link backward stencil kicks and updates the FP64 state
+            TRY(stout_force_impl(c, coeff, FORCE_KICK_EXP_S64, eps_p, eps_u));
+            swap_links(c);
+            return GF_OK;
+        }
         TRY(force_f32(c, coeff));
         return state64_step(c, 2, eps_p, eps_u);
     }

@@ -20,7 +20,9 @@ inline int grid_for(int V, int block = kBlock) { return (V + block - 1) / block;
 inline int grid_sites_mu(int V) { return (V + 63) / 64; }
 
 // force kernel modes
-enum ForceMode { FORCE_STORE = 0, FORCE_KICK = 1, FORCE_KICK_EXP = 2 };
+enum ForceMode { FORCE_STORE = 0, FORCE_KICK = 1, FORCE_KICK_EXP = 2, FORCE_KICK_S64 = 3, FORCE_KICK_EXP_S64 = 4 };
+// FORCE_KICK_S64 / FORCE_KICK_EXP_S64 (FP32 kernels over an FP64 MD state): the FP32 force kicks the
+// FP64 momenta, and the FP64 links are updated in place, their float rounding written to Unew
 
 // ---- k_gauge.hip ----
 // dst = src, bytes a multiple of 16 (streaming copy: the measured HBM ceiling)
@@ -103,7 +105,7 @@ template <typename T>
 // TA(U_in Din), or FP += eps of that, and with FORCE_KICK_EXP also Unew = exp(eps_u FP) U_in.
 hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc,
                                  double rho, hipStream_t st, int fmode = -1, T* FP = nullptr, double eps = 0.0,
-                                 T* Unew = nullptr, double eps_u = 0.0);
+                                 T* Unew = nullptr, double eps_u = 0.0, double* U64 = nullptr);
 // the site-local half of the stout backward alone (Cst null: C recomputed)
 template <typename T>
 hipError_t launch_stout_back_local(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc, double rho,
@@ -111,7 +113,7 @@ hipError_t launch_stout_back_local(const Geo& g, const T* Uin, const T* Cst, con
 // the stencil half of the stout backward alone (fmode as above)
 template <typename T>
 hipError_t launch_stout_back_stencil(const Geo& g, const T* Uin, const T* Lc, T* Din, double rho, hipStream_t st,
-                                     int fmode, T* FP, double eps, T* Unew, double eps_u);
+                                     int fmode, T* FP, double eps, T* Unew, double eps_u, double* U64 = nullptr);
 // stored-C mode: the stencil half of level l (Uin, Lc; Din in: its site-local part) chained with
 // the site-local half of level l-1 (Uprev, Cprev): Din out = level l-1's site-local part, Lout =
 // Lam_{l-1} (a buffer other than Lc)

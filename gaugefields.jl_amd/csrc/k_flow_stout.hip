@@ -363,11 +363,13 @@ __device__ __forceinline__ M3<T> back_stencil_din(const T* __restrict__ Din, uns
 // level of a stout force (test/HMCstout_test_nowing.jl:99-118): Din is not stored but turned into
 // the force FP = -(1/3) TA(U_a(y) Din), or FP += eps of that, in the same kernel; FORCE_KICK_EXP
 // then also writes Unew = exp(eps_u FP) U_a(y) (update_gaugefields!, molecular_dynamics.jl:513-531)
-// to a buffer other than U, which the other threads still read.
+// to a buffer other than U, which the other threads still read.  OUT = FORCE_KICK_S64 /
+// FORCE_KICK_EXP_S64 (FP32 only): FP is the FP64 momenta, kicked by the force as rounded to float;
+// U64 the FP64 links, U64_a(y) <- exp(eps_u FP) U64_a(y) in place, and Unew its float rounding.
 template <typename T, int ST, int OUT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_stout_back_stencil(
     Geo g, const T* __restrict__ U, const T* __restrict__ Lc, T* __restrict__ Din, double rho,
-    T* __restrict__ FP, double eps, T* __restrict__ Unew, double eps_u) {
+    T* __restrict__ FP, double eps, T* __restrict__ Unew, double eps_u, double* __restrict__ U64) {
     __shared__ T2<T> ul[ST ? 4 : 1][6][64];
     __shared__ T2<T> hl[ST ? 4 : 1][4][64];
     const SiteMu sm = site_mu_index(g);
@@ -395,6 +397,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 #pragma unroll
             for (int k = 0; k < 8; k++) c[k] = scale1(fac, c[k]);
             stp<T, GF_ST_NT>(FP, V, a, y, c);
+        } else if constexpr (OUT == FORCE_KICK_S64 || OUT == FORCE_KICK_EXP_S64) {
+            double* P64 = reinterpret_cast<double*>(FP);
+            double p[8];
+            ldp<double, GF_LD_NT>(P64, V, a, y, p);
+#pragma unroll
+            for (int k = 0; k < 8; k++) p[k] = fma(eps, (double)scale1(fac, c[k]), p[k]);
+            stp<double, GF_ST_NT>(P64, V, a, y, p);
+            if constexpr (OUT == FORCE_KICK_EXP_S64) {
+                const M3<double> un = exp_coeffs_times(p, eps_u, ldu<double, GF_LD_NT>(U64, V, a, y));
+                stu<double, GF_ST_NT>(U64, V, a, y, un);
+                stu<T, GF_ST_NT>(Unew, V, a, y, round_link<T>(un));
+            }
         } else {
             T p[8];
             ldp<T, GF_LD_NT>(FP, V, a, y, p);
@@ -522,13 +536,13 @@ hipError_t launch_stout_forward(const Geo& g, const T* Uin, T* Uout, T* Cst, dou
 }
 template <typename T, int OUT>
 void launch_back_stencil(int form, int nb, hipStream_t st, const Geo& g, const T* Uin, const T* Lc, T* Din,
-                         double rho, T* FP, double eps, T* Unew, double eps_u) {
+                         double rho, T* FP, double eps, T* Unew, double eps_u, double* U64) {
     if (form == ST_XROW)
-        k_stout_back_stencil<T, ST_XROW, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
+        k_stout_back_stencil<T, ST_XROW, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u, U64);
     else if (form == ST_SHARED)
-        k_stout_back_stencil<T, ST_SHARED, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
+        k_stout_back_stencil<T, ST_SHARED, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u, U64);
     else
-        k_stout_back_stencil<T, ST_OWN, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
+        k_stout_back_stencil<T, ST_OWN, OUT><<<nb, kBlock, 0, st>>>(g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u, U64);
 }
 template <typename T>
 hipError_t launch_stout_back_chain(const Geo& g, const T* Uin, const T* Lc, T* Din, double rho, const T* Uprev,
@@ -568,35 +582,46 @@ hipError_t launch_stout_back_local(const Geo& g, const T* Uin, const T* Cst, con
 // K6b alone (the thin-link level, or any level whose lower neighbour is not chained)
 template <typename T>
 hipError_t launch_stout_back_stencil(const Geo& g, const T* Uin, const T* Lc, T* Din, double rho, hipStream_t st,
-                                     int fmode, T* FP, double eps, T* Unew, double eps_u) {
+                                     int fmode, T* FP, double eps, T* Unew, double eps_u, double* U64) {
     const int nb = grid_sites_mu(g.V), form = stencil_form(g);
     if (fmode == FORCE_STORE)
-        launch_back_stencil<T, FORCE_STORE>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
+        launch_back_stencil<T, FORCE_STORE>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u, U64);
     else if (fmode == FORCE_KICK)
-        launch_back_stencil<T, FORCE_KICK>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
+        launch_back_stencil<T, FORCE_KICK>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u, U64);
     else if (fmode == FORCE_KICK_EXP)
-        launch_back_stencil<T, FORCE_KICK_EXP>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
-    else
-        launch_back_stencil<T, -1>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u);
+        launch_back_stencil<T, FORCE_KICK_EXP>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u, U64);
+    else if (fmode == FORCE_KICK_S64 || fmode == FORCE_KICK_EXP_S64) {
+        if constexpr (sizeof(T) == 4) {  // FP32 force kernels only
+            if (!U64 && fmode == FORCE_KICK_EXP_S64) return hipErrorInvalidValue;
+            if (fmode == FORCE_KICK_S64)
+                launch_back_stencil<T, FORCE_KICK_S64>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u, U64);
+            else
+                launch_back_stencil<T, FORCE_KICK_EXP_S64>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u,
+                                                           U64);
+        } else {
+            return hipErrorInvalidValue;
+        }
+    } else
+        launch_back_stencil<T, -1>(form, nb, st, g, Uin, Lc, Din, rho, FP, eps, Unew, eps_u, U64);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_stout_backward(const Geo& g, const T* Uin, const T* Cst, const T* Dout, T* Din, T* Lc, double rho,
-                                 hipStream_t st, int fmode, T* FP, double eps, T* Unew, double eps_u) {
+                                 hipStream_t st, int fmode, T* FP, double eps, T* Unew, double eps_u, double* U64) {
     hipError_t e = launch_stout_back_local<T>(g, Uin, Cst, Dout, Din, Lc, rho, st);
     if (e != hipSuccess) return e;
-    return launch_stout_back_stencil<T>(g, Uin, Lc, Din, rho, st, fmode, FP, eps, Unew, eps_u);
+    return launch_stout_back_stencil<T>(g, Uin, Lc, Din, rho, st, fmode, FP, eps, Unew, eps_u, U64);
 }
 
 #define GF_INST2(T)                                                                                       \
     template hipError_t launch_flow_stage<T>(const Geo&, const T*, T*, T*, double, double, hipStream_t);  \
     template hipError_t launch_stout_forward<T>(const Geo&, const T*, T*, T*, double, hipStream_t);       \
     template hipError_t launch_stout_backward<T>(const Geo&, const T*, const T*, const T*, T*, T*, double, \
-                                                 hipStream_t, int, T*, double, T*, double);                \
+                                                 hipStream_t, int, T*, double, T*, double, double*);       \
     template hipError_t launch_stout_back_local<T>(const Geo&, const T*, const T*, const T*, T*, T*, double, \
                                                    hipStream_t);                                            \
     template hipError_t launch_stout_back_stencil<T>(const Geo&, const T*, const T*, T*, double, hipStream_t, \
-                                                     int, T*, double, T*, double);                          \
+                                                     int, T*, double, T*, double, double*);                 \
     template hipError_t launch_stout_back_chain<T>(const Geo&, const T*, const T*, T*, double, const T*,      \
                                                    const T*, T*, hipStream_t);                              \
     template hipError_t launch_dsdu_chain<T>(const Geo&, const T*, double, const T*, const T*, T*, T*, hipStream_t);

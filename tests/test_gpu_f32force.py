@@ -140,6 +140,31 @@ def test_stout_trajectory_and_reversibility(Lat, oracle):
     assert maxdiff(L.download_momenta(), P) < 1e-5
 
 
+@pytest.mark.parametrize("nlev", [0, 3])
+def test_fused_update_equals_kick_then_update(Lat, oracle, nlev):
+    # the fused kernel (force, kick and link update in one launch for the stout path) does the
+    # same double-precision kick and update as the separate calls: identical results
+    print(f"[f32force] fused md_update nlev {nlev}", flush=True)
+    U = alg_links(oracle, D4, 0.3, 81)
+    P = gen.gaussian_momenta(D4, 82)
+    out = []
+    for fused in (True, False):
+        L = Lat()
+        L.upload_links(U)
+        L.upload_momenta(P)
+        if nlev:
+            L.stout_config(nlev, 0.1, "store")
+        for _ in range(2):
+            if fused:
+                L.md_update(COEFF, 0.05, 0.04)
+            else:
+                L.force_kick(COEFF, 0.05)
+                L.exp_update(0.04)
+        out.append((L.download_links(), L.download_momenta(), L.plaquette_sum()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]
+
+
 def test_cold_start(Lat):
     L = Lat()
     H = L.md_trajectory(COEFF, 3, 1.0, "QPQ")
