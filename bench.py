@@ -303,7 +303,7 @@ def stout_hmc_thermalized(gf, dims=DIMS, ntherm=6, npairs=8):
     }
 
 
-def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232, nf32=16):
+def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232):
     """Which rounding sets the FP32-bulk dH deviation (3-level stout HMC, rho 0.12, 2MN 16 steps,
     tau 1, 32^4).  Thermalization: nforce FP64 trajectories accepted unconditionally from
     omega ~ N(0, 0.3^2) links, then nmetro with Metropolis (acceptance reported).  Each of ndraws
@@ -315,9 +315,9 @@ def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232, n
                            projection in double, as the FP32 link updates do);
       round_links_noproj   the same without the projection (links projected only at the start);
       round_both           round_links and round_momenta;
-      fp32_forces          FP64 links and momenta, every force evaluated by an FP32-bulk context
-                           (the whole stout chain in FP32) from the current links rounded to float
-                           (through the host; the first nf32 draws only).
+      f32force             the FP32-force form (GF_F32_FORCE): FP64 links and momenta, every force
+                           (the whole stout chain) evaluated in FP32 from the links rounded to float;
+                           fused steps.
     The FP64 variants step 2MN from the host (kick, update, kick, update, kick per step) with
     gf_round_to_float between the steps; the reference dH is the same unrounded sequence."""
     gen = gf.generators
@@ -343,18 +343,16 @@ def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232, n
     plaq0 = L64.plaquette()
     L32 = gf.Lattice(dims, precision="f32")
     L32.stout_config(3, 0.12, "store")
+    L3 = gf.Lattice(dims, precision="f32force")
+    L3.stout_config(3, 0.12, "store")
 
-    def stepped(rl, rm, project=True, f32force=False):
+    def stepped(rl, rm, project=True):
         H0 = L64.hamiltonian(COEFF)
         eps = 1.0 / 16
         for _ in range(16):
             for kind, h in (("p", lam * eps), ("u", eps / 2), ("p", (1 - 2 * lam) * eps), ("u", eps / 2),
                             ("p", lam * eps)):
-                if kind == "p" and f32force:
-                    L32.upload_links(L64.download_links())
-                    L32.stout_force(COEFF)
-                    L64.upload_momenta(L64.download_momenta() + h * L32.download_force())
-                elif kind == "p":
+                if kind == "p":
                     L64.force_kick(COEFF, h)
                     if rm:
                         L64.round_to_float(momenta=True)
@@ -366,7 +364,7 @@ def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232, n
 
     variants = {"round_momenta": (False, True, True), "round_links": (True, False, True),
                 "round_links_noproj": (True, False, False), "round_both": (True, True, True)}
-    dev = {k: [] for k in ["fp32_bulk", "fp64_fused"] + list(variants) + ["fp32_forces"]}
+    dev = {k: [] for k in ["fp32_bulk", "fp64_fused"] + list(variants) + ["f32force"]}
     ref = []
     t0 = time.perf_counter()
     for d in range(ndraws):
@@ -386,13 +384,13 @@ def precision_study(gf, dims=DIMS, ndraws=48, nforce=20, nmetro=10, seed=3232, n
         L32.upload_links(U0)
         L32.upload_momenta(P0)
         dev["fp32_bulk"].append(L32.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True)[2] - dh64)
-        if d < nf32:
-            L64.upload_links(U0)
-            L64.upload_momenta(P0)
-            dev["fp32_forces"].append(stepped(False, False, f32force=True) - dh64)
+        L3.upload_links(U0)
+        L3.upload_momenta(P0)
+        dev["f32force"].append(L3.md_trajectory(COEFF, 16, 1.0, "2MN", fused=True)[2] - dh64)
         log(f"[precision] draw {d}: dH64 {dh64:.5g} " + " ".join(f"{k} {v[-1]:+.2e}" for k, v in dev.items() if v))
     L64.close()
     L32.close()
+    L3.close()
 
     def stats(x):
         x = np.asarray(x)

@@ -193,8 +193,8 @@ __device__ __forceinline__ void buf_st(float2 v, __amdgpu_buffer_rsrc_t r, unsig
 }
 // FP32 fields keep complex elements (2q, 2q+1) of a site together in one 16-byte slot of "quad
 // plane" q (a last odd element in a pair plane after them): one 64-lane load moves two elements,
-// half the vector-memory instructions of pair planes, and the texture addresser, not HBM, bounds
-// the stencil kernels.  FP64 elements are 16 bytes already and stay in pair planes.
+// half the vector-memory instructions of pair planes (the stencil kernels issue many loads per
+// byte they need from HBM).  FP64 elements are 16 bytes already and stay in pair planes.
 template <int AUX = GF_LD_DEFAULT>
 __device__ __forceinline__ void buf_ld_quad(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float2& a,
                                             float2& b) {
