@@ -61,6 +61,7 @@ const libgfhip = joinpath(@__DIR__, "..", "libgfhip.so")
 
 const GF_F64 = Cint(0)
 const GF_F32 = Cint(1)
+const GF_F32_FORCE = Cint(2)  # FP64 links and momenta, every force evaluated in FP32
 const GF_EINVAL = Cint(-1)
 const GF_QPQ = Cint(0)
 const GF_PQP = Cint(1)
@@ -172,7 +173,8 @@ end
 
 # element type of the stored links: md_driver converts its step sizes to its real type
 # (molecular_dynamics.jl:425-465)
-_elementtype(c::HIPContext) = c.precision == GF_F64 ? ComplexF64 : ComplexF32
+# (an FP32-force context holds FP64 links: its element type is ComplexF64)
+_elementtype(c::HIPContext) = c.precision == GF_F32 ? ComplexF32 : ComplexF64
 Base.eltype(u::Gaugefields_4D_HIP) = _elementtype(u.ctx)
 
 Base.size(u::Gaugefields_4D_HIP) = (u.NC, u.NC, u.NX, u.NY, u.NZ, u.NT)
@@ -181,16 +183,20 @@ Base.size(u::Gaugefields_4D_HIP) = (u.NC, u.NC, u.NX, u.NY, u.NZ, u.NT)
 
 """
     gauge_configuration(lattice, backend::HIPBackend; colors=3, halo=1, start=:cold, seed=nothing,
-                        process_grid=nothing, comm=nothing, boundary=:periodic, eltype=ComplexF64, kwargs...)
+                        process_grid=nothing, comm=nothing, boundary=:periodic, eltype=ComplexF64,
+                        force_eltype=eltype, kwargs...)
 
 The HIP method behind `gauge_configuration(lattice; backend=HIPBackend(), ...)` (API.jl:178-251;
 INTEGRATION.md gives the one-line forward the keyword method needs).  `start=:hot` draws the
 hot start on the device from the Philox4x32-10 stream of `seed` (a random seed if `nothing`).
-`halo` is accepted and ignored: the device layout has no halo.
+`halo` is accepted and ignored: the device layout has no halo.  `force_eltype=ComplexF32` with
+`eltype=ComplexF64` (a HIP-only keyword) keeps the links and momenta in FP64 and evaluates every
+force in FP32 (GF_F32_FORCE).
 """
 function gauge_configuration(lattice::NTuple{4,<:Integer}, backend::HIPBackend; colors::Integer=3,
                              halo::Integer=1, start::Symbol=:cold, seed=nothing, process_grid=nothing,
-                             comm=nothing, boundary=:periodic, eltype::Type=ComplexF64, kwargs...)
+                             comm=nothing, boundary=:periodic, eltype::Type=ComplexF64,
+                             force_eltype::Type=eltype, kwargs...)
     all(>(0), lattice) || throw(ArgumentError("all lattice extents must be positive; got $lattice"))
     colors == 3 || throw(ArgumentError("HIPBackend implements SU(3); got colors=$colors"))
     start in (:cold, :hot) || throw(ArgumentError("start must be :cold or :hot; got $start"))
@@ -201,6 +207,9 @@ function gauge_configuration(lattice::NTuple{4,<:Integer}, backend::HIPBackend; 
         throw(ArgumentError("HIPBackend supports periodic boundaries only"))
     prec = eltype == ComplexF64 ? GF_F64 : eltype == ComplexF32 ? GF_F32 :
            throw(ArgumentError("eltype must be ComplexF64 or ComplexF32; got $eltype"))
+    force_eltype in (eltype, ComplexF32) ||
+        throw(ArgumentError("force_eltype must be the links' eltype or ComplexF32; got $force_eltype"))
+    prec == GF_F64 && force_eltype == ComplexF32 && (prec = GF_F32_FORCE)
     c = HIPContext(ntuple(i -> Int(lattice[i]), 4), prec, backend.device)
     p = _ctx(c)  # gf_create leaves the links at the identity (cold start)
     if start === :hot
