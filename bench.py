@@ -423,17 +423,19 @@ def stout_force_timing(gf, dims=DIMS, nforce=20):
     return out
 
 
-def production_stout_hmc(gf, dims=(48, 48, 48, 48), ntraj=50, check_every=10, seed=4848, ntherm=0):
+def production_stout_hmc(gf, dims=(48, 48, 48, 48), ntraj=50, check_every=10, seed=4848, ntherm=0,
+                         precision="f32"):
     """Production-size stout HMC: SU(3) 48^4, beta 6, 3 stout levels at rho 0.12, Omelyan 2MN
     16 steps, tau 1; FP32 links, momenta and forces (FP64 action/dH sums, Metropolis, CH
-    coefficients); start U = exp(i omega.T), omega ~ N(0, 0.3^2); momenta refreshed on the device
+    coefficients) -- or, with precision "f32force", FP64 links and momenta with FP32 forces;
+    start U = exp(i omega.T), omega ~ N(0, 0.3^2); momenta refreshed on the device
     (Philox stream (seed, trajectory)).  ntherm thermalization trajectories are accepted
     unconditionally first (momentum streams (seed, 10^6 + k)).  Every check_every-th trajectory
     is rerun in FP64 from the same links and momenta (dH error); the last trajectory is run back
     with tau -> -tau from its end point (reversibility |U(tau -> -tau) - U0|)."""
     gen = gf.generators
     t_all = time.perf_counter()
-    L32 = gf.Lattice(dims, precision="f32")
+    L32 = gf.Lattice(dims, precision=precision)
     L32.links_from_algebra(gen.algebra_coefficients(dims, 0.3, seed))
     L32.stout_config(3, 0.12, "auto")
     L64 = gf.Lattice(dims, precision="f64")
@@ -478,7 +480,8 @@ def production_stout_hmc(gf, dims=(48, 48, 48, 48), ntraj=50, check_every=10, se
     mode = L32.stout_mode()
     L32.close()
     L64.close()
-    return {"lattice": "x".join(map(str, dims)), "sites": int(np.prod(dims)), "precision": "fp32 bulk",
+    return {"lattice": "x".join(map(str, dims)), "sites": int(np.prod(dims)),
+            "precision": {"f32": "fp32 bulk", "f32force": "fp32 forces, fp64 links and momenta"}[precision],
             "levels": 3, "rho": 0.12, "integrator": "2MN", "steps": 16, "tau": 1.0,
             "thermalization_dH": therm, "trajectories": ntraj,
             "stout_memory_mode": mode, "s_per_traj_mean": float(np.mean(times)),
@@ -510,6 +513,8 @@ def main():
                     help="run only the 48^4 FP32-bulk stout HMC with NTRAJ trajectories")
     ap.add_argument("--production-therm", type=int, default=0, metavar="N",
                     help="unconditionally accepted thermalization trajectories before --production")
+    ap.add_argument("--production-precision", default="f32", choices=["f32", "f32force"],
+                    help="the FP32 form of --production (FP32 bulk, or FP32 forces over an FP64 state)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path "
@@ -525,7 +530,8 @@ def main():
         return
     if args.production:
         print(json.dumps({"production_stout_hmc": production_stout_hmc(gf, ntraj=args.production,
-                                                                       ntherm=args.production_therm)}))
+                                                                       ntherm=args.production_therm,
+                                                                       precision=args.production_precision)}))
         return
     if args.next_only:  # profiling aid: only the widened rows
         print(json.dumps({"next_rows": next_rows(gf)}))
