@@ -459,8 +459,9 @@ __device__ __forceinline__ void stp(T* __restrict__ p, unsigned V, int mu, unsig
 __device__ __forceinline__ int sel4(int i, int a0, int a1, int a2, int a3) {
     return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
 }
-// LOWER: the lower staples only (links 3..5 of blocks J0..2) and, with OWN, U_mu(x)
-template <typename T, bool OWN, bool LOWER = false, int J0 = 0>
+// LOWER: the lower staples only (links 3..5 of blocks J0..2) and, with OWN, U_mu(x).
+// DEPTH: links in flight ahead of the one being used (1; 2 where the registers allow)
+template <typename T, bool OWN, bool LOWER = false, int J0 = 0, int DEPTH = 1>
 __device__ __forceinline__ M3<T> staple_sigma_pipe(const Geo& g, const T* __restrict__ U, unsigned V, int s,
                                                    const int x[4], int mu, R6<T>* own) {
     int fw[4], bw[4];
@@ -492,12 +493,15 @@ __device__ __forceinline__ M3<T> staple_sigma_pipe(const Geo& g, const T* __rest
     auto slot = [](int i) { return LOWER ? (i < NLOW ? 6 * (i / 3 + J0) + 3 + i % 3 : 18) : i; };
     M3<T> sig = zero3<T>();
     M3<T> X;
-    R6<T> A, nxt = issue(slot(0));
+    R6<T> A, buf[DEPTH];
+#pragma unroll
+    for (int d = 0; d < DEPTH; d++)
+        if (d < NL) buf[d] = issue(slot(d));
 #pragma unroll
     for (int i = 0; i < NL; i++) {
         const int k = slot(i);
-        const R6<T> cur = nxt;
-        if (i + 1 < NL) nxt = issue(slot(i + 1));
+        const R6<T> cur = buf[i % DEPTH];
+        if (i + DEPTH < NL) buf[i % DEPTH] = issue(slot(i + DEPTH));
         __builtin_amdgcn_sched_barrier(0);
         if (k == 18) {
             *own = cur;
@@ -515,10 +519,10 @@ __device__ __forceinline__ M3<T> staple_sigma_pipe(const Geo& g, const T* __rest
     return sig;
 }
 // the staple sum and U_mu(x) (own, as loaded): force, flow and stout-forward kernels
-template <typename T>
+template <typename T, int DEPTH = 1>
 __device__ __forceinline__ M3<T> staple_sigma_own(const Geo& g, const T* __restrict__ U, unsigned V, int s,
                                                   const int x[4], int mu, R6<T>& own) {
-    return staple_sigma_pipe<T, true>(g, U, V, s, x, mu, &own);
+    return staple_sigma_pipe<T, true, false, 0, DEPTH>(g, U, V, s, x, mu, &own);
 }
 
 // ---------------------------------------------------------------------------
