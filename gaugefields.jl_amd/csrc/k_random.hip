@@ -298,9 +298,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T
         const int s = colour_site(g, h, par, x);
         const unsigned V = g.V;
         R6<T> own;
-        // FP64: three staple links in flight ahead of the one in use (two waves per SIMD leave the
-        // registers; 1.37 -> 1.33 ms per heatbath sweep, 1.42 -> 1.31 ms per overrelaxation sweep)
-        M3<double> St = to_f64(staple_sigma_own<T, sizeof(T) == 8 ? 3 : 1>(g, (const T*)U, V, s, x, mu, own));
+        // three staple links in flight ahead of the one in use: the sampler, not the staple, sets
+        // the register peak (FP64 at two waves per SIMD, FP32 at three and four); 32^4 heatbath /
+        // overrelaxation sweep FP64 1.37 / 1.42 -> 1.33 / 1.31 ms, FP32 0.79 / 0.76 -> 0.76 / 0.67 ms
+        M3<double> St = to_f64(staple_sigma_own<T, 3>(g, (const T*)U, V, s, x, mu, own));
         // pin the staple sum: otherwise the compiler interleaves the staple loads with the
         // sampler and needs 256 VGPRs
 #pragma unroll
