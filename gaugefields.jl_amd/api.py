@@ -691,8 +691,7 @@ class SmearedGaugeAction:
 
     def potential(self, U):
         self.configure(U)
-        U.lattice.stout_smear(download=False)
-        # potential of the top smeared level with zero momenta contribution
+        # the Hamiltonian smears the links itself (in FP64); minus the kinetic part
         return U.lattice.hamiltonian(self.coefficient) - 0.5 * U.lattice.momenta_dot()
 
 

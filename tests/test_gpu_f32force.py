@@ -189,3 +189,28 @@ def test_gauge_configuration_force_eltype(gf):
     assert U.lattice.precision == "f32force" and U.eltype == np.complex128
     with pytest.raises(api.ArgumentError):
         api.gauge_configuration((4, 4, 4, 4), eltype=np.complex64, force_eltype=np.complex128)
+
+
+def test_api_smeared_potential_and_trajectory(gf):
+    # the reference-style API on an FP32-force configuration: the smeared potential is the FP64
+    # one of the same links, and a stout trajectory runs through md_trajectory
+    from gaugefields_jl_amd import api
+
+    print("[f32force] api smeared potential", flush=True)
+    U = api.gauge_configuration(D4, start="hot", seed=91, force_eltype=np.complex64)
+    U64 = api.gauge_configuration(D4)
+    U64.lattice.upload_links(U.to_host())
+
+    def smeared(W):
+        a = api.GaugeAction(W)
+        plaq = api.make_loops_fromname("plaquette")
+        a.push(2.5, plaq + api.dagger(plaq))
+        return api.SmearedGaugeAction(a, api.stout_smearing(W, rho=0.1, levels=2, mode="store"))
+
+    s32, s64 = smeared(U), smeared(U64)
+    v32, v64 = api.md_potential(s32, U), api.md_potential(s64, U64)
+    assert abs(v32 - v64) < 1e-12 * abs(v64)
+    P = api.gaussian_momenta(U, seed=92)
+    r = api.md_trajectory(U, P, api.md_driver(U, s32, steps=3, trajectory_length=0.3,
+                                              integrator=api.OmelyanPQP()))
+    assert np.isfinite(r["delta_hamiltonian"])
