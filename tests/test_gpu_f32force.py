@@ -122,16 +122,19 @@ def test_trajectory_against_fp64_oracle(Lat, oracle, integ, fused):
     assert maxdiff(L.download_links(), Uo) < 1e-5 and maxdiff(L.download_momenta(), Po) < 1e-4
 
 
-def test_stout_trajectory_and_reversibility(Lat, oracle):
-    print("[f32force] stout trajectory", flush=True)
-    U = alg_links(oracle, D4, 0.3, 79)
-    P = gen.gaussian_momenta(D4, 80)
-    L = Lat()
+# the three stencil forms of the fused stout epilogue: whole x-rows (4^4), LDS-shared pairs
+# (12,4,4,4), per-thread staples (2,4,2,6: V not a multiple of 64)
+@pytest.mark.parametrize("dims", [D4, (12, 4, 4, 4), (2, 4, 2, 6)])
+def test_stout_trajectory_and_reversibility(Lat, oracle, dims):
+    print(f"[f32force] stout trajectory {dims}", flush=True)
+    U = alg_links(oracle, dims, 0.3, 79)
+    P = gen.gaussian_momenta(dims, 80)
+    L = Lat(dims)
     L.upload_links(U)
     L.upload_momenta(P)
     L.stout_config(3, 0.1, "store")
     H = L.md_trajectory(COEFF, 4, 0.2, "2MN", fused=True)
-    Href = oracle.md_trajectory(D4, U.copy(), P.copy(), COEFF, 4, 0.2, "2MN", nlev=3, rho=0.1)
+    Href = oracle.md_trajectory(dims, U.copy(), P.copy(), COEFF, 4, 0.2, "2MN", nlev=3, rho=0.1)
     assert abs(H[0] - Href[0]) < 1e-12 * abs(Href[0])
     assert abs(H[2] - Href[2]) < 1e-4
     # forward and back: the FP64 state returns to the start up to the FP32 forces' rounding
