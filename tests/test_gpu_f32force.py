@@ -122,6 +122,24 @@ def test_trajectory_against_fp64_oracle(Lat, oracle, integ, fused):
     assert maxdiff(L.download_links(), Uo) < 1e-5 and maxdiff(L.download_momenta(), Po) < 1e-4
 
 
+# the plain force in its other two stencil forms (LDS-shared pairs, per-thread staples), with the
+# FP64 kick and link update of the streaming state kernel
+@pytest.mark.parametrize("dims", [(12, 4, 4, 4), (2, 4, 2, 6)])
+def test_trajectory_other_stencil_forms(Lat, oracle, dims):
+    print(f"[f32force] trajectory {dims}", flush=True)
+    U = alg_links(oracle, dims, 0.3, 83)
+    P = gen.gaussian_momenta(dims, 84)
+    L = Lat(dims)
+    L.upload_links(U)
+    L.upload_momenta(P)
+    H = L.md_trajectory(COEFF, 6, 0.6, "2MN", fused=True)
+    Uo, Po = U.copy(), P.copy()
+    Href = oracle.md_trajectory(dims, Uo, Po, COEFF, 6, 0.6, "2MN")
+    assert abs(H[0] - Href[0]) < 1e-12 * abs(Href[0])
+    assert abs(H[2] - Href[2]) < 1e-4
+    assert maxdiff(L.download_links(), Uo) < 1e-5 and maxdiff(L.download_momenta(), Po) < 1e-4
+
+
 # the three stencil forms of the fused stout epilogue: whole x-rows (4^4), LDS-shared pairs
 # (12,4,4,4), per-thread staples (2,4,2,6: V not a multiple of 64)
 @pytest.mark.parametrize("dims", [D4, (12, 4, 4, 4), (2, 4, 2, 6)])
