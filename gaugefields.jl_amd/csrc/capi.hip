@@ -355,9 +355,10 @@ int stout_force_impl(gf_ctx* c, double coeff, int mode, double eps, double eps_u
 
 // ---- FP32 forces over an FP64 MD state ---------------------------------------------------
 size_t link64_bytes(const gf_ctx* c) { return (size_t)48 * c->V * sizeof(double); }
-int state64_step(gf_ctx* c, int mode, double eps_p, double eps_u) {
-    HIPCHK(c, launch_state64_step(mode, c->geo, (const float*)c->F, (double*)c->P64, (double*)c->U64, (float*)c->U,
-                                  eps_p, eps_u, c->stream));
+// mode 1: U64 <- exp(eps_u P64) U64 and U <- its float rounding; 3: U <- the float rounding of U64
+int state64_step(gf_ctx* c, int mode, double eps_u) {
+    HIPCHK(c, launch_state64_step(mode, c->geo, (const double*)c->P64, (double*)c->U64, (float*)c->U, eps_u,
+                                  c->stream));
     return GF_OK;
 }
 // the force of the links rounded to float, FP32 throughout, into F
@@ -379,8 +380,9 @@ int not_state64(gf_ctx* c, const char* what) {
 int kick(gf_ctx* c, double coeff, double eps) {
     if (c->s64) {
         if (c->nlev > 0) return stout_force_impl(c, coeff, FORCE_KICK_S64, eps);
-        TRY(force_f32(c, coeff));
-        return state64_step(c, 0, eps, 0.0);
+        HIPCHK(c, launch_force<float>(FORCE_KICK_S64, c->geo, (const float*)c->U, (float*)c->P64, nullptr, coeff, eps,
+                                      0.0, c->stream, (double*)c->U64));
+        return GF_OK;
     }
     if (c->nlev > 0) return stout_force_impl(c, coeff, FORCE_KICK, eps);
     return dispatch(c, [&](auto* tag) -> int {
@@ -391,7 +393,7 @@ int kick(gf_ctx* c, double coeff, double eps) {
 }
 
 int expupd(gf_ctx* c, double eps) {
-    if (c->s64) return state64_step(c, 1, 0.0, eps);
+    if (c->s64) return state64_step(c, 1, eps);
     return dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;
         HIPCHK(c, launch_exp_update<T>(c->geo, (const T*)c->U, (T*)c->U, (const T*)c->P, eps, c->stream));
@@ -407,8 +409,11 @@ int kick_exp(gf_ctx* c, double coeff, double eps_p, double eps_u) {
             swap_links(c);
             return GF_OK;
         }
-        TRY(force_f32(c, coeff));
-        return state64_step(c, 2, eps_p, eps_u);
+        // the force kernel kicks P64 and updates U64 in its epilogue (float rounding into Ualt)
+        HIPCHK(c, launch_force<float>(FORCE_KICK_EXP_S64, c->geo, (const float*)c->U, (float*)c->P64,
+                                      (float*)c->Ualt, coeff, eps_p, eps_u, c->stream, (double*)c->U64));
+        swap_links(c);
+        return GF_OK;
     }
     if (c->nlev > 0) {  // the thin-link backward stencil kicks and updates (into Ualt)
         TRY(stout_force_impl(c, coeff, FORCE_KICK_EXP, eps_p, eps_u));
@@ -666,7 +671,7 @@ int gf_upload_links(gf_ctx* c, const double* ref) {
     HIPCHK(c, hipMemcpyAsync(c->staging, ref, (size_t)72 * c->V * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if (c->s64) {
         HIPCHK(c, launch_links_from_ref<double>(c->staging, (double*)c->U64, c->V, false, c->stream));
-        TRY(state64_step(c, 3, 0.0, 0.0));
+        TRY(state64_step(c, 3, 0.0));
     } else {
         TRY(dispatch(c, [&](auto* tag) -> int {
             using T = std::remove_pointer_t<decltype(tag)>;
@@ -790,7 +795,7 @@ int gf_restore_links(gf_ctx* c) {
     if (c->s64) {
         if (!c->U64bak) return fail(c, GF_ESTATE, "gf_restore_links before gf_save_links");
         HIPCHK(c, hipMemcpyAsync(c->U64, c->U64bak, link64_bytes(c), hipMemcpyDeviceToDevice, c->stream));
-        return state64_step(c, 3, 0.0, 0.0);
+        return state64_step(c, 3, 0.0);
     }
     if (!c->Ubak) return fail(c, GF_ESTATE, "gf_restore_links before gf_save_links");
     HIPCHK(c, hipMemcpyAsync(c->U, c->Ubak, link_bytes(c), hipMemcpyDeviceToDevice, c->stream));
@@ -955,7 +960,7 @@ int gf_hot_start(gf_ctx* c, unsigned long long seed) {
     CTXCHK(c);
     if (c->s64) {
         HIPCHK(c, launch_hot_links<double>(c->geo, (double*)c->U64, seed, c->stream));
-        return state64_step(c, 3, 0.0, 0.0);
+        return state64_step(c, 3, 0.0);
     }
     return dispatch(c, [&](auto* tag) -> int {
         using T = std::remove_pointer_t<decltype(tag)>;

@@ -42,17 +42,20 @@ hipError_t launch_dsdu(const Geo& g, const T* U, double coeff, T* D, hipStream_t
 // MODE FORCE_STORE: F = -(coeff/3) TA(U Sigma)
 // MODE FORCE_KICK : P += eps_p * F
 // MODE FORCE_KICK_EXP : P += eps_p * F;  Unew = exp(eps_u P) U
+// MODE FORCE_KICK_S64, FORCE_KICK_EXP_S64 (float): the same on the FP64 momenta (FP) and links
+//   (U64) of an FP64 state, with Unew the updated links rounded to float
 template <typename T>
 hipError_t launch_force(int mode, const Geo& g, const T* U, T* FP, T* Unew, double coeff, double eps_p,
-                        double eps_u, hipStream_t st);
+                        double eps_u, hipStream_t st, double* U64 = nullptr);
 template <typename T>
 hipError_t launch_exp_update(const Geo& g, const T* Uin, T* Uout, const T* P, double eps, hipStream_t st);
 // reference-layout (column-major 3x3 per link) <-> device field; general = false: SU(3) link
 // field (rows 0 and 1 kept, row 2 rebuilt on the way out), true: general matrix field
 hipError_t launch_links_widen(const float* Uf, double* Ud, int V, hipStream_t st);
-// FP32 forces over an FP64 MD state (k_state64_step): mode 0 kick, 1 exp update, 2 both, 3 rounding
-hipError_t launch_state64_step(int mode, const Geo& g, const float* F, double* P64, double* U64, float* U32,
-                               double eps_p, double eps_u, hipStream_t st);
+// FP32 forces over an FP64 MD state (k_state64_step): mode 1 exp update of U64 (and its float
+// rounding into U32), 3 rounding only
+hipError_t launch_state64_step(int mode, const Geo& g, const double* P64, double* U64, float* U32, double eps_u,
+                               hipStream_t st);
 template <typename T>
 hipError_t launch_links_from_ref(const double* ref, T* soa, int V, bool general, hipStream_t st);
 template <typename T>

@@ -186,12 +186,17 @@ __global__ __launch_bounds__(kBlock) void k_dsdu(Geo g, const T* __restrict__ U,
 //   FORCE_STORE    : FP = F
 //   FORCE_KICK     : FP += eps_p F                    (update_momenta!, :539-551)
 //   FORCE_KICK_EXP : FP += eps_p F; Unew = exp(eps_u FP) U   (then update_gaugefields!, :513-531)
+//   FORCE_KICK_S64, FORCE_KICK_EXP_S64 (float only: FP32 forces over an FP64 MD state): the same
+//     on the FP64 momenta (FP points to them) and FP64 links U64 (updated in place: a thread reads
+//     only its own U64 link); the float force enters as F = float(fac c), as if stored, and Unew
+//     receives the updated links rounded to float
 // ST (stencil_form): ST_OWN per-thread staples; ST_SHARED upper-staple plaquettes shared between
 // directions through LDS (ta_force_shared, V % 64 == 0); ST_XROW the same with the x-neighbour
 // links from LDS as well (ta_force_xrow, NX divides 64).  At most 168 VGPRs: three waves per SIMD.
 template <typename T, int MODE, int ST>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_force(Geo g, const T* __restrict__ U, T* __restrict__ FP,
-                                                  T* __restrict__ Unew, double fac, double eps_u) {
+                                                  T* __restrict__ Unew, double fac, double eps_u,
+                                                  double eps_p, double* __restrict__ U64) {
     __shared__ T2<T> ol[ST ? 4 : 1][6][64];
     __shared__ T2<T> tp[ST ? 7 : 1][4][64];
     const SiteMu sm = site_mu_index(g);
@@ -212,7 +217,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         ta_coeffs(mul(full_of(own), sig), c);
     }
     const M3<T> umu = full_of(own);
-    if (MODE == FORCE_STORE) {
+    if constexpr (MODE == FORCE_KICK_S64 || MODE == FORCE_KICK_EXP_S64) {
+        double* P64 = reinterpret_cast<double*>(FP);
+        double p[8];
+        ldp<double, GF_LD_NT>(P64, V, mu, s, p);
+#pragma unroll
+        for (int a = 0; a < 8; a++) p[a] = fma(eps_p, (double)scale1(fac, c[a]), p[a]);
+        stp<double, GF_ST_NT>(P64, V, mu, s, p);
+        if constexpr (MODE == FORCE_KICK_EXP_S64) {
+            const M3<double> un = exp_coeffs_times(p, eps_u, ldu<double, GF_LD_NT>(U64, V, mu, s));
+            stu<double, GF_ST_NT>(U64, V, mu, s, un);
+            stu<T, GF_ST_NT>(Unew, V, mu, s, round_link<T>(un));
+        }
+    } else if (MODE == FORCE_STORE) {
 #pragma unroll
         for (int a = 0; a < 8; a++) c[a] = scale1(fac, c[a]);
         stp<T, GF_ST_NT>(FP, V, mu, s, c);
@@ -266,15 +283,15 @@ __global__ __launch_bounds__(kBlock) void k_links_widen(const float* __restrict_
     }
 }
 
-// FP32 forces over an FP64 MD state: the links U64 and momenta P64 are held and updated in double;
-// the force chain reads U32, the links rounded to float, and leaves its force in F (float).
-// MODE 0: P64 += eps_p F.  1: U64 <- exp(eps_u P64) U64 (in place: only the thread's own link is
-// read) and U32 <- round(U64).  2: both (kick, then update).  3: U32 <- round(U64).
+// FP32 forces over an FP64 MD state: the links U64 and momenta P64 are held and updated in double,
+// the force kernels read U32, the links rounded to float (their FP64-state epilogues kick P64 and
+// update U64 in the same launch).  The lone steps: MODE 1: U64 <- exp(eps_u P64) U64 (in place:
+// only the thread's own link is read) and U32 <- round(U64).  MODE 3: U32 <- round(U64).
 // One thread per site walking its four links: a streaming kernel.
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_state64_step(Geo g, const float* __restrict__ F, double* __restrict__ P64,
+__global__ __launch_bounds__(kBlock) void k_state64_step(Geo g, const double* __restrict__ P64,
                                                          double* __restrict__ U64, float* __restrict__ U32,
-                                                         double eps_p, double eps_u) {
+                                                         double eps_u) {
     const int s = site_index();
     if (s >= g.V) return;
     const unsigned V = g.V;
@@ -286,30 +303,21 @@ __global__ __launch_bounds__(kBlock) void k_state64_step(Geo g, const float* __r
         }
         double p[8];
         ldp<double, GF_LD_NT>(P64, V, mu, s, p);
-        if (MODE != 1) {
-            float f[8];
-            ldp<float, GF_LD_NT>(F, V, mu, s, f);
-#pragma unroll
-            for (int a = 0; a < 8; a++) p[a] = fma(eps_p, (double)f[a], p[a]);
-            stp<double, GF_ST_NT>(P64, V, mu, s, p);
-        }
-        if (MODE != 0) {
-            const M3<double> un = exp_coeffs_times(p, eps_u, ldu<double, GF_LD_NT>(U64, V, mu, s));
-            stu<double, GF_ST_NT>(U64, V, mu, s, un);
-            stu<float, GF_ST_NT>(U32, V, mu, s, round_link<float>(un));
-        }
+        const M3<double> un = exp_coeffs_times(p, eps_u, ldu<double, GF_LD_NT>(U64, V, mu, s));
+        stu<double, GF_ST_NT>(U64, V, mu, s, un);
+        stu<float, GF_ST_NT>(U32, V, mu, s, round_link<float>(un));
     }
 }
 
-hipError_t launch_state64_step(int mode, const Geo& g, const float* F, double* P64, double* U64, float* U32,
-                               double eps_p, double eps_u, hipStream_t st) {
+hipError_t launch_state64_step(int mode, const Geo& g, const double* P64, double* U64, float* U32, double eps_u,
+                               hipStream_t st) {
     const int nb = grid_for(g.V);
-    switch (mode) {
-        case 0: k_state64_step<0><<<nb, kBlock, 0, st>>>(g, F, P64, U64, U32, eps_p, eps_u); break;
-        case 1: k_state64_step<1><<<nb, kBlock, 0, st>>>(g, F, P64, U64, U32, eps_p, eps_u); break;
-        case 2: k_state64_step<2><<<nb, kBlock, 0, st>>>(g, F, P64, U64, U32, eps_p, eps_u); break;
-        default: k_state64_step<3><<<nb, kBlock, 0, st>>>(g, F, P64, U64, U32, eps_p, eps_u); break;
-    }
+    if (mode == 1)
+        k_state64_step<1><<<nb, kBlock, 0, st>>>(g, P64, U64, U32, eps_u);
+    else if (mode == 3)
+        k_state64_step<3><<<nb, kBlock, 0, st>>>(g, P64, U64, U32, eps_u);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -499,17 +507,17 @@ hipError_t launch_dsdu(const Geo& g, const T* U, double coeff, T* D, hipStream_t
 }
 template <typename T, int MODE>
 void launch_force_st(int form, int nb, hipStream_t st, const Geo& g, const T* U, T* FP, T* Unew, double fac,
-                     double eps_u) {
+                     double eps_u, double eps_p = 0.0, double* U64 = nullptr) {
     if (form == ST_XROW)
-        k_force<T, MODE, ST_XROW><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac, eps_u);
+        k_force<T, MODE, ST_XROW><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac, eps_u, eps_p, U64);
     else if (form == ST_SHARED)
-        k_force<T, MODE, ST_SHARED><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac, eps_u);
+        k_force<T, MODE, ST_SHARED><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac, eps_u, eps_p, U64);
     else
-        k_force<T, MODE, ST_OWN><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac, eps_u);
+        k_force<T, MODE, ST_OWN><<<nb, kBlock, 0, st>>>(g, U, FP, Unew, fac, eps_u, eps_p, U64);
 }
 template <typename T>
 hipError_t launch_force(int mode, const Geo& g, const T* U, T* FP, T* Unew, double coeff, double eps_p,
-                        double eps_u, hipStream_t st) {
+                        double eps_u, hipStream_t st, double* U64) {
     const int nb = grid_sites_mu(g.V);
     const double fac = -coeff / 3.0;
     const int form = stencil_form(g);
@@ -523,6 +531,17 @@ hipError_t launch_force(int mode, const Geo& g, const T* U, T* FP, T* Unew, doub
         case FORCE_KICK_EXP:
             launch_force_st<T, FORCE_KICK_EXP>(form, nb, st, g, U, FP, Unew, fac * eps_p, eps_u);
             break;
+        case FORCE_KICK_S64:
+        case FORCE_KICK_EXP_S64:
+            if constexpr (sizeof(T) == 4) {
+                if (mode == FORCE_KICK_S64)
+                    launch_force_st<T, FORCE_KICK_S64>(form, nb, st, g, U, FP, Unew, fac, 0.0, eps_p, U64);
+                else
+                    launch_force_st<T, FORCE_KICK_EXP_S64>(form, nb, st, g, U, FP, Unew, fac, eps_u, eps_p, U64);
+                break;
+            } else {
+                return hipErrorInvalidValue;
+            }
         default:
             return hipErrorInvalidValue;
     }
@@ -587,7 +606,7 @@ hipError_t launch_axpby(T* y, const T* x, double a, double b, size_t n, hipStrea
     template hipError_t launch_momenta_dot<T>(const Geo&, const T*, double*, double*, hipStream_t);      \
     template hipError_t launch_dsdu<T>(const Geo&, const T*, double, T*, hipStream_t);                   \
     template hipError_t launch_force<T>(int, const Geo&, const T*, T*, T*, double, double, double,       \
-                                        hipStream_t);                                                    \
+                                        hipStream_t, double*);                                           \
     template hipError_t launch_exp_update<T>(const Geo&, const T*, T*, const T*, double, hipStream_t);   \
     template hipError_t launch_links_from_ref<T>(const double*, T*, int, bool, hipStream_t);             \
     template hipError_t launch_links_to_ref<T>(const T*, double*, int, bool, hipStream_t);               \
