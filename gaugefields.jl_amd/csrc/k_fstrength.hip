@@ -37,15 +37,19 @@ __device__ __forceinline__ unsigned site_of(const Geo& g, const int y[4]) {
 
 // product of links along one loop from x (evaluate_gaugelinks!, AbstractGaugefields.jl:1797-1860):
 // (d, +n): U_d(y), y -> y + d, n times; (d, -n): y -> y - d, U_d(y)^dag, n times.
-// Every partial product is SU(3): only its rows 0 and 1 are formed (72 instead of 108 FMAs per
-// product), and row 2 of the loop is rebuilt once at the end; a link entering as the first
-// factor of a positive step needs no row 2 either.  Links are widened to double before row 2 is
-// rebuilt, as in every FP64 sum over FP32-stored links.
+// FP64 links are SU(3) to double rounding, so every partial product is too: only its rows 0 and
+// 1 are formed (72 instead of 108 FMAs per product) and row 2 of the loop is rebuilt once at the
+// end; a link entering as the first factor of a positive step needs no row 2 either.  FP32-stored
+// links are unitary only to float rounding, which the cross product would carry into every
+// partial product: there the full products are formed.  Links are widened to double before
+// row 2 is rebuilt, as in every FP64 sum over FP32-stored links.
 template <typename T>
 __device__ __forceinline__ M3<double> walk(const Geo& g, const T* __restrict__ U, unsigned V, const int x[4],
                                            int mu, int nu, const signed char (&st)[4][2]) {
+    constexpr bool ROWS = sizeof(T) == 8;
     int y[4] = {x[0], x[1], x[2], x[3]};
-    R6<double> r;
+    R6<double> r;   // ROWS: rows 0 and 1 of the product so far
+    M3<double> m;   // otherwise: the product so far
     bool first = true;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -57,18 +61,27 @@ __device__ __forceinline__ M3<double> walk(const Geo& g, const T* __restrict__ U
             if (q < n) {
                 if (len > 0) {
                     const R6<double> l = r6_f64(ldr(U, V, d, site_of(g, y)));
-                    r = first ? l : rows_of(su3_mul(r, full_of(l)));
+                    if constexpr (ROWS)
+                        r = first ? l : rows_of(su3_mul(r, full_of(l)));
+                    else
+                        m = first ? full_of(l) : mul(m, full_of(l));
                     y[d] = y[d] + 1 == g.n[d] ? 0 : y[d] + 1;
                 } else {
                     y[d] = y[d] == 0 ? g.n[d] - 1 : y[d] - 1;
                     const M3<double> l = full_of(r6_f64(ldr(U, V, d, site_of(g, y))));
-                    r = first ? rows_of(dag(l)) : rows_of(su3_mul_nd(r, l));
+                    if constexpr (ROWS)
+                        r = first ? rows_of(dag(l)) : rows_of(su3_mul_nd(r, l));
+                    else
+                        m = first ? dag(l) : mul_nd(m, l);
                 }
                 first = false;
             }
         }
     }
-    return full_of(r);
+    if constexpr (ROWS)
+        return full_of(r);
+    else
+        return m;
 }
 
 __device__ __forceinline__ double pair_sum(const double (*c)[8][64], int lane) {
