@@ -92,6 +92,22 @@ def time_kernel(L, fn, reps, warmup=1):
     return L.event_elapsed_ms(2, 3) / reps
 
 
+def md_update_forms(gf, reps=20):
+    """the fused MD update of the plain Wilson action (one kernel per step in every form) at 32^4,
+    ms per step: FP64, FP32 bulk, FP32 forces over an FP64 state"""
+    out = {}
+    for prec in ("f64", "f32", "f32force"):
+        L = make_lattice(gf, DIMS, prec)
+        fn = lambda: L.md_update(COEFF, 0.02, 0.02)  # noqa: E731
+        for _ in range(100):
+            fn()
+        out[prec] = time_kernel(L, fn, reps)
+        L.close()
+    out["speedup_f32_over_f64"] = out["f64"] / out["f32"]
+    out["speedup_f32force_over_f64"] = out["f64"] / out["f32force"]
+    return out
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -607,6 +623,8 @@ def main():
         log(f"[bench] stout {json.dumps(stout)}")
         flow = flow_compare(gf)
         log(f"[bench] flow {json.dumps(flow)}")
+        kernels["md_update_forms_ms"] = md_update_forms(gf)
+        log(f"[bench] md update forms {json.dumps(kernels['md_update_forms_ms'])}")
     nxt = None
     if not args.no_next:
         nxt = next_rows(gf)
